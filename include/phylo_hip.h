@@ -1,0 +1,122 @@
+/*
+ * phylo_hip.h -- C ABI of libphylo_hip, the MI355X (gfx950) Felsenstein-pruning
+ * likelihood engine that replaces the compute behind phylommand's simpleML.
+ *
+ * The reference path is a statically linked C++ class interface
+ * (simpleML.h:30-142, sub_model.h:28-128, rate_model.h:29-84); its
+ * simpleML-compatible C++ host classes (phylommand_amd/host/) sit above this
+ * ABI and keep those signatures.  Every entry point below names the
+ * reference interface whose work it takes over.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only; every host buffer is caller-owned and
+ *    copied before the call returns.  The library owns its device memory and
+ *    its HIP stream (one stream per context).
+ *  - int status: 0 = ok, negative = error; phylo_last_error() returns text.
+ *  - One context per thread; calls on one context are not reentrant.
+ *  - Nodes: 0..n_taxa-1 are the tips (node i carries taxon row i of the tip
+ *    masks); nodes n_taxa..n_nodes-1 are internal with left >= 0 and
+ *    right >= 0, or right == -1 for a single-child node (the reference keeps
+ *    such nodes, simpleML.cpp:170-182).
+ *  - Tip states are bit masks, bit i = state i (the reference's
+ *    bitset<SIZE> per site, character_vector.h:88); ambiguity sets several
+ *    bits, gaps/missing set all bits (treeator.cpp:463-466).
+ */
+#ifndef PHYLO_HIP_H
+#define PHYLO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct phylo_ctx phylo_ctx;
+
+enum {
+    PHYLO_OK = 0,
+    PHYLO_ERR_ARG = -1,         /* bad argument (size, index, value) */
+    PHYLO_ERR_STATE = -2,       /* a required set_* call is missing */
+    PHYLO_ERR_HIP = -3,         /* HIP runtime error or no GPU */
+    PHYLO_ERR_DATA = -4,        /* a tip has no state at some site (check_nodes, simpleML.cpp:52-71) */
+    PHYLO_ERR_UNSUPPORTED = -5  /* combination not implemented */
+};
+
+enum { PHYLO_PREC_FP32 = 0, PHYLO_PREC_FP64 = 1 };
+
+/* P(t) builders.  REFCOMPAT is the reference's (I + Q t / 2^20)^(2^20)
+ * (marth.cpp:129-140 via sub_model.cpp:115-127); EIGEN is V exp(L t) V^-1
+ * for reversible models; PADE is scaling-and-squaring Pade for any Q. */
+enum { PHYLO_P_REFCOMPAT = 0, PHYLO_P_EIGEN = 1, PHYLO_P_PADE = 2 };
+
+/* Library / context life cycle.  Replaces: construction of simpleML and its
+ * CLV store (simpleML.h:33-40, init_nodes simpleML.cpp:24-31). */
+int phylo_create(int device, phylo_ctx **out);
+void phylo_destroy(phylo_ctx *ctx);
+const char *phylo_last_error(const phylo_ctx *ctx);
+int phylo_version(void);
+
+/* PHYLO_PREC_FP32 (default): partials stored in FP32 with per-site
+ * power-of-two scaling, FP64 arithmetic; PHYLO_PREC_FP64: partials stored in FP64. */
+int phylo_set_precision(phylo_ctx *ctx, int precision);
+
+/* Tip states for all taxa and sites.  masks: [n_taxa][n_sites] row-major.
+ * Replaces: simpleML::set_char (simpleML.h:41-57), one call per (taxon, site). */
+int phylo_set_tips(phylo_ctx *ctx, int n_taxa, int n_sites, int n_states, const uint32_t *masks);
+
+/* Optional per-site weights (site-pattern counts); NULL restores weight 1.
+ * The reference's MLtree.h:44-47 sketches the same pattern compression. */
+int phylo_set_site_weights(phylo_ctx *ctx, const double *weights);
+
+/* Rooted topology.  Replaces: the tree that simpleML inherits
+ * (tree_file_parser, tree.cpp:69-237) as the pruning order. */
+int phylo_set_topology(phylo_ctx *ctx, int n_nodes, int root, const int32_t *left, const int32_t *right);
+
+/* Effective length of the edge above each node (n_nodes values; the root's
+ * is ignored), already transformed by the host rate model.  Replaces:
+ * rate_model::get_time_mod_branch_length x get_rate_clade
+ * (rate_model.cpp:103-127, rate_model.h:64-67, applied at simpleML.cpp:173,178). */
+int phylo_set_edge_lengths(phylo_ctx *ctx, const double *eff_len);
+
+/* Substitution model.  Q: n_states x n_states row-major rate matrix as
+ * sub_model::calc_Q_matrix builds it (sub_model.cpp:99-114).  root_weights:
+ * per-state weights of the root sum -- pi for time-reversible models, all ones
+ * for the reference's non-reversible models (simpleML.h:132-133).  pi: the
+ * stationary frequencies, required by PHYLO_P_EIGEN (may be NULL otherwise).
+ * Replaces: sub_model::set_P_matrix / get_P_value (sub_model.cpp:115-127,
+ * sub_model.h:46-50) and the root weighting (simpleML.h:123-141). */
+int phylo_set_model(phylo_ctx *ctx, int n_states, const double *Q, const double *root_weights,
+                    const double *pi, int p_builder);
+
+/* Rate categories: branch-length multipliers and mixture weights (1 category
+ * = the reference's single-rate model).  New in this library: the reference
+ * has no rate heterogeneity. */
+int phylo_set_categories(phylo_ctx *ctx, int n_cat, const double *rates, const double *weights);
+
+/* One full postorder evaluation; all internal-node partials are kept on the
+ * device.  site_lnl (nullable): n_sites values.  total: weighted sum of the
+ * site log-likelihoods (compensated FP64 sum).
+ * Replaces: simpleML::calculate_log_likelihood(sub_model&[, rate_model&])
+ * (simpleML.h:58-90), calculate_likelihood (simpleML.cpp:127-184),
+ * branch_likelihood (:186-197), get_log_likelihood_from_root (simpleML.h:123-141). */
+int phylo_loglik(phylo_ctx *ctx, double *site_lnl, double *total);
+
+/* Partials of one internal node from the last phylo_loglik: clv
+ * [n_sites][n_cat][n_states] (scaled), scale [n_sites] base-2 exponents so
+ * that the true partial is clv * 2^scale.  Replaces: reading simpleML's
+ * likelihoods map in draw_normalized_likelihood_on_nodes (simpleML.cpp:199-231). */
+int phylo_node_partials(phylo_ctx *ctx, int node, double *clv, int32_t *scale);
+
+/* P matrices the library built for the edge above `node`: [n_cat][S][S].
+ * Replaces: sub_model::get_P_value after set_P_matrix (sub_model.h:46-50). */
+int phylo_get_pmatrix(phylo_ctx *ctx, int node, double *P);
+
+/* Device time (ms, HIP events on the library's stream) of the pruning
+ * kernel in the last phylo_loglik / phylo_nni_score call. */
+int phylo_last_kernel_ms(phylo_ctx *ctx, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHYLO_HIP_H */

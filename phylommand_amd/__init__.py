@@ -1,0 +1,180 @@
+"""phylommand_amd: MI355X-native Felsenstein-pruning likelihood for phylommand's
+treeator path.
+
+The compute lives in libphylo_hip.so (hand-written gfx950 HIP kernels behind
+the C ABI declared in include/phylo_hip.h).  This module is a thin ctypes
+binding over that ABI, used by the tests and the benchmark; the
+simpleML/sub_model/rate_model-compatible C++ host classes are in
+phylommand_amd/host/.  There is no CPU fallback: if the library is missing or
+no GPU is present, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libphylo_hip.so")
+
+PHYLO_OK = 0
+PREC_FP32, PREC_FP64 = 0, 1
+P_REFCOMPAT, P_EIGEN, P_PADE = 0, 1, 2
+
+_lib = None
+
+# (name, restype, argtypes) of every entry point in include/phylo_hip.h
+_c_int, _c_dbl_p, _c_int_p, _c_void_p = ctypes.c_int, ctypes.POINTER(ctypes.c_double), \
+    ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p
+SIGNATURES = [
+    ("phylo_version", _c_int, []),
+    ("phylo_create", _c_int, [_c_int, ctypes.POINTER(_c_void_p)]),
+    ("phylo_destroy", None, [_c_void_p]),
+    ("phylo_last_error", ctypes.c_char_p, [_c_void_p]),
+    ("phylo_set_precision", _c_int, [_c_void_p, _c_int]),
+    ("phylo_set_tips", _c_int, [_c_void_p, _c_int, _c_int, _c_int, ctypes.POINTER(ctypes.c_uint32)]),
+    ("phylo_set_site_weights", _c_int, [_c_void_p, _c_dbl_p]),
+    ("phylo_set_topology", _c_int, [_c_void_p, _c_int, _c_int, _c_int_p, _c_int_p]),
+    ("phylo_set_edge_lengths", _c_int, [_c_void_p, _c_dbl_p]),
+    ("phylo_set_model", _c_int, [_c_void_p, _c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_int]),
+    ("phylo_set_categories", _c_int, [_c_void_p, _c_int, _c_dbl_p, _c_dbl_p]),
+    ("phylo_loglik", _c_int, [_c_void_p, _c_dbl_p, _c_dbl_p]),
+    ("phylo_node_partials", _c_int, [_c_void_p, _c_int, _c_dbl_p, _c_int_p]),
+    ("phylo_get_pmatrix", _c_int, [_c_void_p, _c_int, _c_dbl_p]),
+    ("phylo_last_kernel_ms", _c_int, [_c_void_p, _c_dbl_p]),
+]
+
+
+class PhyloError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libphylo_hip error {code}: {msg}")
+        self.code = code
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libphylo_hip.so and declares every exported symbol.  Raises if the
+    library has not been built (python -m phylommand_amd.build)."""
+    global _lib
+    if _lib is not None and path == LIB_PATH:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is missing: build it with `python -m phylommand_amd.build`")
+    lib = ctypes.CDLL(path)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path == LIB_PATH:
+        _lib = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_c_dbl_p)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_c_int_p)
+
+
+class Context:
+    """One libphylo_hip context (its own HIP stream and device buffers)."""
+
+    def __init__(self, device: int = 0, precision: int = PREC_FP32):
+        self.lib = load_library()
+        h = _c_void_p()
+        rc = self.lib.phylo_create(device, ctypes.byref(h))
+        if rc != PHYLO_OK:
+            raise PhyloError(rc, "phylo_create failed (no GPU visible?)")
+        self.h = h
+        self._check(self.lib.phylo_set_precision(self.h, precision))
+        self.n_sites = 0
+        self.n_nodes = 0
+
+    def _check(self, rc):
+        if rc != PHYLO_OK:
+            raise PhyloError(rc, self.lib.phylo_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.phylo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_precision(self, precision):
+        self._check(self.lib.phylo_set_precision(self.h, precision))
+
+    def set_tips(self, masks: np.ndarray, n_states: int):
+        masks = np.ascontiguousarray(masks, dtype=np.uint32)
+        n_taxa, m = masks.shape
+        self.n_sites = m
+        self._check(self.lib.phylo_set_tips(self.h, n_taxa, m, n_states,
+                                            masks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+
+    def set_site_weights(self, w):
+        if w is None:
+            self._check(self.lib.phylo_set_site_weights(self.h, None))
+        else:
+            w = np.ascontiguousarray(w, dtype=np.float64)
+            self._check(self.lib.phylo_set_site_weights(self.h, _dp(w)))
+
+    def set_topology(self, left, right, root: int):
+        left = np.ascontiguousarray(left, dtype=np.int32)
+        right = np.ascontiguousarray(right, dtype=np.int32)
+        self.n_nodes = len(left)
+        self._check(self.lib.phylo_set_topology(self.h, len(left), int(root), _ip(left), _ip(right)))
+
+    def set_edge_lengths(self, eff):
+        eff = np.ascontiguousarray(eff, dtype=np.float64)
+        self._check(self.lib.phylo_set_edge_lengths(self.h, _dp(eff)))
+
+    def set_model(self, Q, root_weights, pi=None, builder: int = P_EIGEN):
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        rw = np.ascontiguousarray(root_weights, dtype=np.float64)
+        pp = None if pi is None else _dp(np.ascontiguousarray(pi, dtype=np.float64))
+        self._pi_keep = None if pi is None else np.ascontiguousarray(pi, dtype=np.float64)
+        if pi is not None:
+            pp = _dp(self._pi_keep)
+        self._check(self.lib.phylo_set_model(self.h, Q.shape[0], _dp(Q), _dp(rw), pp, builder))
+
+    def set_categories(self, rates, weights=None):
+        rates = np.ascontiguousarray(rates, dtype=np.float64)
+        if weights is None:
+            weights = np.full(len(rates), 1.0 / len(rates))
+        weights = np.ascontiguousarray(weights, dtype=np.float64)
+        self._check(self.lib.phylo_set_categories(self.h, len(rates), _dp(rates), _dp(weights)))
+
+    def loglik(self, per_site: bool = False):
+        tot = np.zeros(1)
+        site = np.zeros(self.n_sites) if per_site else None
+        self._check(self.lib.phylo_loglik(self.h, _dp(site) if per_site else None, _dp(tot)))
+        return (float(tot[0]), site) if per_site else float(tot[0])
+
+    def node_partials(self, node: int, n_cat: int, n_states: int):
+        clv = np.zeros((self.n_sites, n_cat, n_states))
+        sc = np.zeros(self.n_sites, np.int32)
+        self._check(self.lib.phylo_node_partials(self.h, int(node), _dp(clv), _ip(sc)))
+        return clv, sc
+
+    def pmatrix(self, node: int, n_cat: int, n_states: int):
+        P = np.zeros((n_cat, n_states, n_states))
+        self._check(self.lib.phylo_get_pmatrix(self.h, int(node), _dp(P)))
+        return P
+
+    def last_kernel_ms(self) -> float:
+        ms = np.zeros(1)
+        self._check(self.lib.phylo_last_kernel_ms(self.h, _dp(ms)))
+        return float(ms[0])

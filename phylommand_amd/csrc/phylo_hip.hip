@@ -1,0 +1,784 @@
+// phylo_hip.hip -- libphylo_hip: Felsenstein pruning on MI355X (gfx950).
+//
+// Design
+//  * A likelihood evaluation is compiled on the host into a "pruning
+//    program": the internal nodes in the reference's postorder (left subtree,
+//    right subtree, node; simpleML.cpp:127-184), each op naming where its two
+//    child partials come from (a tip row, a stored partial, or a register
+//    carry holding the result of an earlier op of the same program).
+//  * One launch walks the whole program.  Each work-item owns one site and
+//    runs every op for it, so no inter-workgroup dependency exists and all
+//    postorder levels are fused into the launch.  The partial of the op just
+//    computed stays in registers; when the next op consumes it, it is not
+//    read back.
+//  * Partials are stored site-major and state-interleaved:
+//    clv[slot][cat][site][4] (one 16-byte float4 per work-item per category),
+//    so every global load/store of a wavefront is one contiguous 1 KiB burst.
+//    DNA tips are one byte per (taxon, site) holding the 4-bit state mask.
+//  * Arithmetic is FP64; storage is FP32 (default) or FP64.  P matrices are
+//    FP64, indexed by the child node and category, and are wave-uniform so
+//    the compiler keeps them in scalar registers.
+//  * Every node's partial is renormalised by an exact power of two (frexp /
+//    ldexp) so its largest entry is in [0.5, 1); the integer exponent is
+//    accumulated into one scaler per (node, site) shared by all categories.
+//  * Per-site lnL = log(sum_c w_c sum_i rw_i L_ci) + scaler*ln2 in FP64,
+//    reduced per workgroup with wavefront shuffles, then summed in a fixed
+//    order with compensation by a second tiny launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/phylo_hip.h"
+#include "model_math.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxCat = 8;
+constexpr int kMaxStates = 32;
+constexpr double kLn2 = 0.69314718055994530942;
+
+enum SrcKind : int { SRC_TIP = 0, SRC_CLV = 1, SRC_CARRY_A = 2, SRC_CARRY_B = 3, SRC_NONE = 4 };
+enum OpFlags : int { OP_ROOT = 1, OP_TO_B = 2 };
+
+struct alignas(16) Op {
+    int l_node, r_node;   // child node ids: P matrices are per (child node, category)
+    int l_kind, r_kind;   // SrcKind
+    int l_idx, r_idx;     // tip row or partial slot
+    int out_slot;         // slot to store the result in, -1 for none
+    int flags;            // OpFlags
+};
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+
+__device__ inline void load4(const float* p, double v[4]) {
+    const float4 x = *reinterpret_cast<const float4*>(p);
+    v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+}
+__device__ inline void load4(const double* p, double v[4]) {
+    const double2 a = reinterpret_cast<const double2*>(p)[0];
+    const double2 b = reinterpret_cast<const double2*>(p)[1];
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+__device__ inline void store4(float* p, const float v[4]) {
+    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ inline void store4(double* p, const double v[4]) {
+    reinterpret_cast<double2*>(p)[0] = make_double2(v[0], v[1]);
+    reinterpret_cast<double2*>(p)[1] = make_double2(v[2], v[3]);
+}
+
+template <typename T, int C>
+__device__ inline void child_partial(int kind, int idx, int node, int s, int m_pad,
+                                     const uint8_t* __restrict__ tips, const T* clv, const int* scl,
+                                     const double* __restrict__ P, const T (&cA)[C][4], int sA,
+                                     const T (&cB)[C][4], int sB, double (&out)[C][4], int& sc) {
+    const double* Pn = P + (size_t)node * C * 16;
+    if (kind == SRC_TIP) {
+        const unsigned code = tips[(size_t)idx * m_pad + s];
+        double b[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = (code >> j) & 1u ? 1.0 : 0.0;
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                double a = 0.0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) a = fma(Pn[c * 16 + i * 4 + j], b[j], a);
+                out[c][i] = a;
+            }
+        return;
+    }
+    double L[C][4];
+    if (kind == SRC_CLV) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) load4(clv + (((size_t)idx * C + c) * m_pad + s) * 4, L[c]);
+        sc += scl[(size_t)idx * m_pad + s];
+    } else if (kind == SRC_CARRY_A) {
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) L[c][i] = (double)cA[c][i];
+        sc += sA;
+    } else {
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) L[c][i] = (double)cB[c][i];
+        sc += sB;
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            double a = 0.0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) a = fma(Pn[c * 16 + i * 4 + j], L[c][j], a);
+            out[c][i] = a;
+        }
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ inline void block_sum_store(double v, double* dst) {
+    __shared__ double red[kBlock / 64];
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) t += red[w];
+        *dst = t;
+    }
+}
+
+// The pruning walker for S <= 4 (DNA, binary; padded states carry zero rows/cols of P).
+// grid = (m_pad / kBlock, n_programs); program y runs ops[prog_begin[y] .. prog_begin[y+1]).
+template <typename T, int C>
+__global__ __launch_bounds__(kBlock) void prune4_kernel(
+    const Op* __restrict__ ops, const int* __restrict__ prog_begin, const uint8_t* __restrict__ tips,
+    T* clv, int* scl, const double* __restrict__ P, const double* __restrict__ roots,
+    const double* __restrict__ site_w, double* __restrict__ site_lnl, double* __restrict__ partial,
+    int m, int m_pad) {
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    const int prog = blockIdx.y;
+    const int k0 = prog_begin[prog], k1 = prog_begin[prog + 1];
+    T cA[C][4], cB[C][4];
+    int sA = 0, sB = 0;
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) { cA[c][i] = T(0); cB[c][i] = T(0); }
+    double lnl = 0.0;
+    for (int k = k0; k < k1; ++k) {
+        const Op op = ops[k];
+        double vl[C][4], vr[C][4];
+        int sc = 0;
+        child_partial<T, C>(op.l_kind, op.l_idx, op.l_node, s, m_pad, tips, clv, scl, P, cA, sA, cB, sB, vl, sc);
+        if (op.r_kind != SRC_NONE) {
+            child_partial<T, C>(op.r_kind, op.r_idx, op.r_node, s, m_pad, tips, clv, scl, P, cA, sA, cB, sB, vr, sc);
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) vl[c][i] *= vr[c][i];
+        }
+        double mx = 0.0;
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) mx = fmax(mx, vl[c][i]);
+        int e = 0;
+        if (mx > 0.0) (void)frexp(mx, &e);
+        sc += e;
+        T r[C][4];
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                vl[c][i] = ldexp(vl[c][i], -e);
+                r[c][i] = (T)vl[c][i];
+            }
+        if (op.out_slot >= 0) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) store4(clv + (((size_t)op.out_slot * C + c) * m_pad + s) * 4, r[c]);
+            scl[(size_t)op.out_slot * m_pad + s] = sc;
+        }
+        if (op.flags & OP_TO_B) {
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) cB[c][i] = r[c][i];
+            sB = sc;
+        } else {
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) cA[c][i] = r[c][i];
+            sA = sc;
+        }
+        if (op.flags & OP_ROOT) {
+            double like = 0.0;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                double lc = 0.0;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) lc = fma(roots[i], vl[c][i], lc);
+                like = fma(roots[4 + c], lc, like);
+            }
+            lnl = log(like) + sc * kLn2;
+        }
+    }
+    const bool live = s < m;
+    if (live && site_lnl) site_lnl[(size_t)prog * m + s] = lnl;
+    const double w = live ? (site_w ? site_w[s] : 1.0) : 0.0;
+    block_sum_store(live ? w * lnl : 0.0, partial + (size_t)prog * gridDim.x + blockIdx.x);
+}
+
+// Compensated fixed-order sum of the per-workgroup partials of each program.
+__global__ void final_sum_kernel(const double* __restrict__ partial, int n_part, int n_prog, double* __restrict__ out) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= n_prog) return;
+    double s = 0.0, comp = 0.0;
+    for (int i = 0; i < n_part; ++i) {
+        const double x = partial[(size_t)y * n_part + i] - comp;
+        const double t = s + x;
+        comp = (t - s) - x;
+        s = t;
+    }
+    out[y] = s;
+}
+
+// P[node][cat][i][j] = sum_k V[i][k] exp(lam_k r_c t_node) Vinv[k][j], padded to Sp x Sp with zeros.
+__global__ void pbuild_eigen_kernel(const double* __restrict__ V, const double* __restrict__ Vinv,
+                                    const double* __restrict__ lam, const double* __restrict__ eff,
+                                    const double* __restrict__ cat_rate, int S, int Sp, int C, int n_nodes,
+                                    double* __restrict__ P) {
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)n_nodes * C * Sp * Sp;
+    if (gid >= total) return;
+    const int j = gid % Sp;
+    const int i = (gid / Sp) % Sp;
+    const int c = (gid / ((long)Sp * Sp)) % C;
+    const int node = gid / ((long)Sp * Sp * C);
+    double v = 0.0;
+    if (i < S && j < S) {
+        const double t = eff[node] * cat_rate[c];
+        for (int k = 0; k < S; ++k) v = fma(V[i * S + k] * exp(lam[k] * t), Vinv[k * S + j], v);
+    }
+    P[gid] = v;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+struct phylo_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    int prec = PHYLO_PREC_FP32;
+    double last_ms = 0.0;
+
+    // problem
+    int n_taxa = 0, m = 0, S = 0, m_pad = 0;
+    std::vector<uint32_t> masks;
+    bool have_tips = false, dirty_tips = true;
+    std::vector<double> site_w;
+    bool have_site_w = false, dirty_site_w = false;
+    int n_nodes = 0, root = -1;
+    std::vector<int> left, right;
+    bool have_topo = false, dirty_topo = true;
+    std::vector<double> eff;
+    bool have_eff = false;
+    int S_model = 0, builder = PHYLO_P_EIGEN;
+    std::vector<double> Q, root_w, pi, V, Vinv, lam;
+    bool have_model = false;
+    int C = 1;
+    std::vector<double> cat_rate{1.0}, cat_w{1.0};
+    bool dirty_P = true;
+
+    // compiled program
+    std::vector<Op> ops;
+    std::vector<int> prog_begin;
+    int n_slots = 0;
+
+    // device buffers
+    uint8_t* d_tips = nullptr; size_t tips_bytes = 0;
+    void* d_clv = nullptr; size_t clv_bytes = 0;
+    int* d_scl = nullptr; size_t scl_bytes = 0;
+    double* d_P = nullptr; size_t P_bytes = 0;
+    Op* d_ops = nullptr; size_t ops_bytes = 0;
+    int* d_prog = nullptr; size_t prog_bytes = 0;
+    double* d_roots = nullptr;
+    double* d_sitew = nullptr; size_t sitew_bytes = 0;
+    double* d_sitelnl = nullptr; size_t sitelnl_bytes = 0;
+    double* d_partial = nullptr; size_t partial_bytes = 0;
+    double* d_total = nullptr; size_t total_bytes = 0;
+    double* d_eig = nullptr; size_t eig_bytes = 0;
+    double* d_eff = nullptr; size_t eff_bytes = 0;
+    double* h_total = nullptr; size_t h_total_bytes = 0;  // pinned
+};
+
+namespace {
+
+int fail(phylo_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail((ctx), PHYLO_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// (Re)allocates a device buffer when it has to grow.
+template <typename T>
+hipError_t ensure(T*& ptr, size_t& have, size_t need) {
+    if (need <= have && ptr) return hipSuccess;
+    if (ptr) { hipError_t e = hipFree(ptr); if (e != hipSuccess) return e; ptr = nullptr; have = 0; }
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&ptr), need ? need : 16);
+    if (e == hipSuccess) have = need;
+    return e;
+}
+
+int padded_states(int S) { return S <= 4 ? 4 : 32; }
+
+// Builds the full-evaluation program: internal nodes in the reference's
+// postorder, carry A holding the previous op's result.
+int compile_full_program(phylo_ctx* c) {
+    const int n = c->n_nodes;
+    std::vector<int> order;
+    order.reserve(n);
+    std::vector<int> stack{c->root};
+    std::vector<char> seen(n, 0);
+    while (!stack.empty()) {
+        int v = stack.back();
+        if (!seen[v]) {
+            seen[v] = 1;
+            if (c->right[v] >= 0) stack.push_back(c->right[v]);
+            if (c->left[v] >= 0) stack.push_back(c->left[v]);
+        } else {
+            stack.pop_back();
+            if (v >= c->n_taxa) order.push_back(v);
+        }
+    }
+    if ((int)order.size() != n - c->n_taxa)
+        return fail(c, PHYLO_ERR_ARG, "topology: internal nodes unreachable from the root");
+    c->ops.clear();
+    int prev = -1;
+    auto src = [&](int child, int& kind, int& idx) {
+        if (child < 0) { kind = SRC_NONE; idx = 0; }
+        else if (child < c->n_taxa) { kind = SRC_TIP; idx = child; }
+        else if (child == prev) { kind = SRC_CARRY_A; idx = 0; }
+        else { kind = SRC_CLV; idx = child - c->n_taxa; }
+    };
+    for (int v : order) {
+        Op op{};
+        op.l_node = c->left[v];
+        op.r_node = c->right[v] < 0 ? 0 : c->right[v];
+        src(c->left[v], op.l_kind, op.l_idx);
+        src(c->right[v], op.r_kind, op.r_idx);
+        op.out_slot = v - c->n_taxa;
+        op.flags = (v == c->root) ? OP_ROOT : 0;
+        c->ops.push_back(op);
+        prev = v;
+    }
+    c->prog_begin = {0, (int)c->ops.size()};
+    c->n_slots = n - c->n_taxa;
+    return PHYLO_OK;
+}
+
+// Host-built P for REFCOMPAT / PADE, laid out [node][cat][Sp][Sp].
+void host_pmatrices(const phylo_ctx* c, int Sp, std::vector<double>& P) {
+    const int S = c->S_model;
+    P.assign((size_t)c->n_nodes * c->C * Sp * Sp, 0.0);
+    std::vector<double> tmp((size_t)S * S);
+    for (int v = 0; v < c->n_nodes; ++v) {
+        if (v == c->root) continue;
+        for (int k = 0; k < c->C; ++k) {
+            const double t = c->eff[v] * c->cat_rate[k];
+            if (c->builder == PHYLO_P_REFCOMPAT) phylo_math::p_refcompat(S, c->Q.data(), t, tmp.data());
+            else phylo_math::expm_pade13(S, c->Q.data(), t, tmp.data());
+            double* dst = P.data() + ((size_t)v * c->C + k) * Sp * Sp;
+            for (int i = 0; i < S; ++i)
+                for (int j = 0; j < S; ++j) dst[i * Sp + j] = tmp[(size_t)i * S + j];
+        }
+    }
+}
+
+int upload_pmatrices(phylo_ctx* c) {
+    const int Sp = padded_states(c->S_model);
+    const size_t nP = (size_t)c->n_nodes * c->C * Sp * Sp;
+    HIPCHK(c, ensure(c->d_P, c->P_bytes, nP * sizeof(double)));
+    if (c->builder == PHYLO_P_EIGEN) {
+        const int S = c->S_model;
+        std::vector<double> eig;
+        eig.insert(eig.end(), c->V.begin(), c->V.end());
+        eig.insert(eig.end(), c->Vinv.begin(), c->Vinv.end());
+        eig.insert(eig.end(), c->lam.begin(), c->lam.end());
+        eig.insert(eig.end(), c->cat_rate.begin(), c->cat_rate.end());
+        HIPCHK(c, ensure(c->d_eig, c->eig_bytes, eig.size() * sizeof(double)));
+        HIPCHK(c, ensure(c->d_eff, c->eff_bytes, c->eff.size() * sizeof(double)));
+        HIPCHK(c, hipMemcpyAsync(c->d_eig, eig.data(), eig.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_eff, c->eff.data(), c->eff.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        const double* dV = c->d_eig;
+        const double* dVi = dV + (size_t)S * S;
+        const double* dl = dVi + (size_t)S * S;
+        const double* dr = dl + S;
+        const int threads = 256;
+        const long blocks = ((long)nP + threads - 1) / threads;
+        hipLaunchKernelGGL(pbuild_eigen_kernel, dim3((unsigned)blocks), dim3(threads), 0, c->stream, dV, dVi, dl,
+                           c->d_eff, dr, S, Sp, c->C, c->n_nodes, c->d_P);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipStreamSynchronize(c->stream));  // host vectors above go out of scope
+    } else {
+        std::vector<double> P;
+        host_pmatrices(c, Sp, P);
+        HIPCHK(c, hipMemcpyAsync(c->d_P, P.data(), P.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    c->dirty_P = false;
+    return PHYLO_OK;
+}
+
+int prepare(phylo_ctx* c) {
+    if (!c->have_tips) return fail(c, PHYLO_ERR_STATE, "phylo_set_tips has not been called");
+    if (!c->have_topo) return fail(c, PHYLO_ERR_STATE, "phylo_set_topology has not been called");
+    if (!c->have_eff) return fail(c, PHYLO_ERR_STATE, "phylo_set_edge_lengths has not been called");
+    if (!c->have_model) return fail(c, PHYLO_ERR_STATE, "phylo_set_model has not been called");
+    if (c->S_model != c->S) return fail(c, PHYLO_ERR_ARG, "model and tip data disagree on the number of states");
+    if (c->S > 4) return fail(c, PHYLO_ERR_UNSUPPORTED, "more than 4 states needs the wide-state kernel");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->dirty_tips) {
+        // [taxon][m_pad] 4-bit codes; padding sites are all-states (finite, ignored in the sum)
+        std::vector<uint8_t> codes((size_t)c->n_taxa * c->m_pad, 0x0F);
+        for (int t = 0; t < c->n_taxa; ++t)
+            for (int s = 0; s < c->m; ++s)
+                codes[(size_t)t * c->m_pad + s] = (uint8_t)(c->masks[(size_t)t * c->m + s] & 0x0Fu);
+        HIPCHK(c, ensure(c->d_tips, c->tips_bytes, codes.size()));
+        HIPCHK(c, hipMemcpyAsync(c->d_tips, codes.data(), codes.size(), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->dirty_tips = false;
+    }
+    if (c->dirty_site_w) {
+        if (c->have_site_w) {
+            HIPCHK(c, ensure(c->d_sitew, c->sitew_bytes, (size_t)c->m * sizeof(double)));
+            HIPCHK(c, hipMemcpyAsync(c->d_sitew, c->site_w.data(), (size_t)c->m * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+        c->dirty_site_w = false;
+    }
+    if (c->dirty_topo) {
+        int rc = compile_full_program(c);
+        if (rc) return rc;
+        HIPCHK(c, ensure(c->d_ops, c->ops_bytes, c->ops.size() * sizeof(Op)));
+        HIPCHK(c, ensure(c->d_prog, c->prog_bytes, c->prog_begin.size() * sizeof(int)));
+        HIPCHK(c, hipMemcpyAsync(c->d_ops, c->ops.data(), c->ops.size() * sizeof(Op), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_prog, c->prog_begin.data(), c->prog_begin.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->dirty_topo = false;
+    }
+    const size_t elt = c->prec == PHYLO_PREC_FP64 ? sizeof(double) : sizeof(float);
+    HIPCHK(c, ensure(c->d_clv, c->clv_bytes, (size_t)std::max(c->n_slots, 1) * c->C * c->m_pad * 4 * elt));
+    HIPCHK(c, ensure(c->d_scl, c->scl_bytes, (size_t)std::max(c->n_slots, 1) * c->m_pad * sizeof(int)));
+    if (c->dirty_P) {
+        int rc = upload_pmatrices(c);
+        if (rc) return rc;
+    }
+    std::vector<double> roots(4 + kMaxCat, 0.0);
+    for (int i = 0; i < c->S; ++i) roots[i] = c->root_w[i];
+    for (int k = 0; k < c->C; ++k) roots[4 + k] = c->cat_w[k];
+    if (!c->d_roots) HIPCHK(c, hipMalloc(&c->d_roots, roots.size() * sizeof(double)));
+    HIPCHK(c, hipMemcpyAsync(c->d_roots, roots.data(), roots.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PHYLO_OK;
+}
+
+template <typename T, int C>
+void launch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
+    hipLaunchKernelGGL((prune4_kernel<T, C>), grid, dim3(kBlock), 0, c->stream, c->d_ops, c->d_prog, c->d_tips,
+                       reinterpret_cast<T*>(c->d_clv), c->d_scl, c->d_P, c->d_roots,
+                       c->have_site_w ? c->d_sitew : nullptr, site_lnl, c->d_partial, c->m, c->m_pad);
+}
+
+template <typename T>
+int dispatch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
+    switch (c->C) {
+    case 1: launch_prune4<T, 1>(c, grid, site_lnl); break;
+    case 2: launch_prune4<T, 2>(c, grid, site_lnl); break;
+    case 4: launch_prune4<T, 4>(c, grid, site_lnl); break;
+    case 8: launch_prune4<T, 8>(c, grid, site_lnl); break;
+    default: return fail(c, PHYLO_ERR_UNSUPPORTED, "number of rate categories must be 1, 2, 4 or 8");
+    }
+    return PHYLO_OK;
+}
+
+// Launches the program set currently in c->ops / c->prog_begin and returns
+// one total per program in totals (n_prog values); site_lnl [n_prog][m] is optional.
+int run_programs(phylo_ctx* c, int n_prog, double* totals, double* site_lnl_host) {
+    const int nblk = c->m_pad / kBlock;
+    HIPCHK(c, ensure(c->d_partial, c->partial_bytes, (size_t)n_prog * nblk * sizeof(double)));
+    HIPCHK(c, ensure(c->d_total, c->total_bytes, (size_t)n_prog * sizeof(double)));
+    if (site_lnl_host) HIPCHK(c, ensure(c->d_sitelnl, c->sitelnl_bytes, (size_t)n_prog * c->m * sizeof(double)));
+    if (c->h_total_bytes < (size_t)n_prog * sizeof(double)) {
+        if (c->h_total) HIPCHK(c, hipHostFree(c->h_total));
+        c->h_total = nullptr;
+        HIPCHK(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_total), (size_t)n_prog * sizeof(double)));
+        c->h_total_bytes = (size_t)n_prog * sizeof(double);
+    }
+    dim3 grid(nblk, n_prog);
+    double* dsite = site_lnl_host ? c->d_sitelnl : nullptr;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    int rc = c->prec == PHYLO_PREC_FP64 ? dispatch_prune4<double>(c, grid, dsite) : dispatch_prune4<float>(c, grid, dsite);
+    if (rc) return rc;
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    hipLaunchKernelGGL(final_sum_kernel, dim3((n_prog + 63) / 64), dim3(64), 0, c->stream, c->d_partial, nblk, n_prog, c->d_total);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->h_total, c->d_total, (size_t)n_prog * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (site_lnl_host)
+        HIPCHK(c, hipMemcpyAsync(site_lnl_host, c->d_sitelnl, (size_t)n_prog * c->m * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    std::memcpy(totals, c->h_total, (size_t)n_prog * sizeof(double));
+    return PHYLO_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int phylo_version(void) { return 1; }
+
+int phylo_create(int device, phylo_ctx** out) {
+    if (!out) return PHYLO_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return PHYLO_ERR_HIP;
+    if (device < 0 || device >= n) return PHYLO_ERR_ARG;
+    phylo_ctx* c = new phylo_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return PHYLO_ERR_HIP;
+    }
+    *out = c;
+    return PHYLO_OK;
+}
+
+void phylo_destroy(phylo_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void* bufs[] = {c->d_tips, c->d_clv, c->d_scl, c->d_P, c->d_ops, c->d_prog, c->d_roots, c->d_sitew,
+                    c->d_sitelnl, c->d_partial, c->d_total, c->d_eig, c->d_eff};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->h_total) (void)hipHostFree(c->h_total);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* phylo_last_error(const phylo_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int phylo_set_precision(phylo_ctx* c, int precision) {
+    if (!c) return PHYLO_ERR_ARG;
+    if (precision != PHYLO_PREC_FP32 && precision != PHYLO_PREC_FP64) return fail(c, PHYLO_ERR_ARG, "precision");
+    if (precision != c->prec) {
+        c->prec = precision;
+        if (c->d_clv) { (void)hipFree(c->d_clv); c->d_clv = nullptr; c->clv_bytes = 0; }
+    }
+    return PHYLO_OK;
+}
+
+int phylo_set_tips(phylo_ctx* c, int n_taxa, int n_sites, int n_states, const uint32_t* masks) {
+    if (!c) return PHYLO_ERR_ARG;
+    if (n_taxa < 2 || n_sites < 1 || n_states < 1 || n_states > kMaxStates || !masks)
+        return fail(c, PHYLO_ERR_ARG, "set_tips: need n_taxa >= 2, n_sites >= 1, 1 <= n_states <= 32");
+    const uint32_t valid = n_states == 32 ? 0xFFFFFFFFu : ((1u << n_states) - 1u);
+    for (size_t i = 0; i < (size_t)n_taxa * n_sites; ++i)
+        if ((masks[i] & valid) == 0u) {
+            // simpleML::check_nodes (simpleML.cpp:52-71): a tip without any state
+            char buf[160];
+            snprintf(buf, sizeof buf, "taxon %zu lacks a character state for character %zu", i / n_sites, i % n_sites);
+            return fail(c, PHYLO_ERR_DATA, buf);
+        }
+    c->n_taxa = n_taxa;
+    c->m = n_sites;
+    c->S = n_states;
+    c->m_pad = (n_sites + kBlock - 1) / kBlock * kBlock;
+    c->masks.assign(masks, masks + (size_t)n_taxa * n_sites);
+    for (auto& x : c->masks) x &= valid;
+    c->have_tips = true;
+    c->dirty_tips = true;
+    c->dirty_topo = true;     // slot numbering depends on n_taxa
+    c->have_site_w = false;
+    c->dirty_site_w = false;
+    return PHYLO_OK;
+}
+
+int phylo_set_site_weights(phylo_ctx* c, const double* w) {
+    if (!c) return PHYLO_ERR_ARG;
+    if (!c->have_tips) return fail(c, PHYLO_ERR_STATE, "set_site_weights before set_tips");
+    if (!w) { c->have_site_w = false; c->dirty_site_w = false; return PHYLO_OK; }
+    c->site_w.assign(w, w + c->m);
+    c->have_site_w = true;
+    c->dirty_site_w = true;
+    return PHYLO_OK;
+}
+
+int phylo_set_topology(phylo_ctx* c, int n_nodes, int root, const int32_t* left, const int32_t* right) {
+    if (!c) return PHYLO_ERR_ARG;
+    if (!c->have_tips) return fail(c, PHYLO_ERR_STATE, "set_topology before set_tips");
+    if (n_nodes <= c->n_taxa || root < c->n_taxa || root >= n_nodes || !left || !right)
+        return fail(c, PHYLO_ERR_ARG, "set_topology: bad node count or root");
+    std::vector<int> indeg(n_nodes, 0);
+    for (int v = 0; v < n_nodes; ++v) {
+        const bool tip = v < c->n_taxa;
+        if (tip && (left[v] != -1 || right[v] != -1)) return fail(c, PHYLO_ERR_ARG, "set_topology: tips must have no children");
+        if (!tip) {
+            if (left[v] < 0 || left[v] >= n_nodes || right[v] < -1 || right[v] >= n_nodes || left[v] == right[v] ||
+                left[v] == v || right[v] == v)
+                return fail(c, PHYLO_ERR_ARG, "set_topology: bad child index");
+            ++indeg[left[v]];
+            if (right[v] >= 0) ++indeg[right[v]];
+        }
+    }
+    for (int v = 0; v < n_nodes; ++v)
+        if ((v == root && indeg[v] != 0) || (v != root && indeg[v] != 1))
+            return fail(c, PHYLO_ERR_ARG, "set_topology: every non-root node needs exactly one parent");
+    c->n_nodes = n_nodes;
+    c->root = root;
+    c->left.assign(left, left + n_nodes);
+    c->right.assign(right, right + n_nodes);
+    c->have_topo = true;
+    c->dirty_topo = true;
+    c->have_eff = false;
+    c->dirty_P = true;
+    return PHYLO_OK;
+}
+
+int phylo_set_edge_lengths(phylo_ctx* c, const double* eff) {
+    if (!c) return PHYLO_ERR_ARG;
+    if (!c->have_topo) return fail(c, PHYLO_ERR_STATE, "set_edge_lengths before set_topology");
+    if (!eff) return fail(c, PHYLO_ERR_ARG, "set_edge_lengths: null");
+    for (int v = 0; v < c->n_nodes; ++v)
+        if (v != c->root && !std::isfinite(eff[v])) return fail(c, PHYLO_ERR_ARG, "set_edge_lengths: non-finite length");
+    c->eff.assign(eff, eff + c->n_nodes);
+    c->eff[c->root] = 0.0;
+    c->have_eff = true;
+    c->dirty_P = true;
+    return PHYLO_OK;
+}
+
+int phylo_set_model(phylo_ctx* c, int n_states, const double* Q, const double* root_w, const double* pi,
+                    int p_builder) {
+    if (!c) return PHYLO_ERR_ARG;
+    if (n_states < 1 || n_states > kMaxStates || !Q || !root_w) return fail(c, PHYLO_ERR_ARG, "set_model: bad arguments");
+    if (p_builder != PHYLO_P_REFCOMPAT && p_builder != PHYLO_P_EIGEN && p_builder != PHYLO_P_PADE)
+        return fail(c, PHYLO_ERR_ARG, "set_model: unknown P builder");
+    for (int i = 0; i < n_states * n_states; ++i)
+        if (!std::isfinite(Q[i])) return fail(c, PHYLO_ERR_ARG, "set_model: non-finite Q");
+    c->Q.assign(Q, Q + (size_t)n_states * n_states);
+    c->root_w.assign(root_w, root_w + n_states);
+    c->S_model = n_states;
+    c->builder = p_builder;
+    if (p_builder == PHYLO_P_EIGEN) {
+        if (!pi) return fail(c, PHYLO_ERR_ARG, "set_model: the eigen builder needs pi");
+        c->pi.assign(pi, pi + n_states);
+        if (!phylo_math::is_reversible(n_states, Q, pi))
+            return fail(c, PHYLO_ERR_ARG, "set_model: Q is not reversible with respect to pi; use PHYLO_P_PADE");
+        if (!phylo_math::reversible_eigen(n_states, Q, pi, c->V, c->lam, c->Vinv))
+            return fail(c, PHYLO_ERR_ARG, "set_model: eigen-decomposition failed (pi must be positive)");
+    }
+    c->have_model = true;
+    c->dirty_P = true;
+    return PHYLO_OK;
+}
+
+int phylo_set_categories(phylo_ctx* c, int n_cat, const double* rates, const double* weights) {
+    if (!c) return PHYLO_ERR_ARG;
+    if (n_cat < 1 || n_cat > kMaxCat || !rates || !weights) return fail(c, PHYLO_ERR_ARG, "set_categories: 1..8 categories");
+    for (int k = 0; k < n_cat; ++k)
+        if (!(rates[k] >= 0.0) || !(weights[k] >= 0.0) || !std::isfinite(rates[k]))
+            return fail(c, PHYLO_ERR_ARG, "set_categories: rates and weights must be finite and >= 0");
+    c->C = n_cat;
+    c->cat_rate.assign(rates, rates + n_cat);
+    c->cat_w.assign(weights, weights + n_cat);
+    c->dirty_P = true;
+    if (c->d_clv) { (void)hipFree(c->d_clv); c->d_clv = nullptr; c->clv_bytes = 0; }
+    return PHYLO_OK;
+}
+
+int phylo_loglik(phylo_ctx* c, double* site_lnl, double* total) {
+    if (!c) return PHYLO_ERR_ARG;
+    if (!total) return fail(c, PHYLO_ERR_ARG, "loglik: total is required");
+    int rc = prepare(c);
+    if (rc) return rc;
+    return run_programs(c, 1, total, site_lnl);
+}
+
+int phylo_node_partials(phylo_ctx* c, int node, double* clv, int32_t* scale) {
+    if (!c) return PHYLO_ERR_ARG;
+    if (!c->d_clv || c->ops.empty()) return fail(c, PHYLO_ERR_STATE, "node_partials before loglik");
+    if (node < c->n_taxa || node >= c->n_nodes) return fail(c, PHYLO_ERR_ARG, "node_partials: not an internal node");
+    const int slot = node - c->n_taxa;
+    const int Sp = 4;
+    const size_t n = (size_t)c->C * c->m_pad * Sp;
+    std::vector<int> sc(c->m_pad);
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->prec == PHYLO_PREC_FP64) {
+        std::vector<double> buf(n);
+        HIPCHK(c, hipMemcpyAsync(buf.data(), reinterpret_cast<double*>(c->d_clv) + slot * n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(sc.data(), c->d_scl + (size_t)slot * c->m_pad, c->m_pad * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int s = 0; s < c->m; ++s)
+            for (int k = 0; k < c->C; ++k)
+                for (int i = 0; i < c->S; ++i)
+                    clv[((size_t)s * c->C + k) * c->S + i] = buf[((size_t)k * c->m_pad + s) * Sp + i];
+    } else {
+        std::vector<float> buf(n);
+        HIPCHK(c, hipMemcpyAsync(buf.data(), reinterpret_cast<float*>(c->d_clv) + slot * n, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(sc.data(), c->d_scl + (size_t)slot * c->m_pad, c->m_pad * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int s = 0; s < c->m; ++s)
+            for (int k = 0; k < c->C; ++k)
+                for (int i = 0; i < c->S; ++i)
+                    clv[((size_t)s * c->C + k) * c->S + i] = buf[((size_t)k * c->m_pad + s) * Sp + i];
+    }
+    if (scale)
+        for (int s = 0; s < c->m; ++s) scale[s] = sc[s];
+    return PHYLO_OK;
+}
+
+int phylo_get_pmatrix(phylo_ctx* c, int node, double* P) {
+    if (!c || !P) return PHYLO_ERR_ARG;
+    if (node < 0 || node >= c->n_nodes) return fail(c, PHYLO_ERR_ARG, "get_pmatrix: node out of range");
+    if (c->dirty_P || !c->d_P) {
+        int rc = prepare(c);
+        if (rc) return rc;
+    }
+    const int Sp = padded_states(c->S_model), S = c->S_model;
+    std::vector<double> buf((size_t)c->C * Sp * Sp);
+    HIPCHK(c, hipMemcpy(buf.data(), c->d_P + (size_t)node * c->C * Sp * Sp, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < c->C; ++k)
+        for (int i = 0; i < S; ++i)
+            for (int j = 0; j < S; ++j) P[((size_t)k * S + i) * S + j] = buf[((size_t)k * Sp + i) * Sp + j];
+    return PHYLO_OK;
+}
+
+int phylo_last_kernel_ms(phylo_ctx* c, double* ms) {
+    if (!c || !ms) return PHYLO_ERR_ARG;
+    *ms = c->last_ms;
+    return PHYLO_OK;
+}
+
+}  // extern "C"
