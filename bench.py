@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Benchmark: Felsenstein-pruning CLV updates/s on one MI355X.
+
+Workload (the headline configuration): GTR+G4 DNA, 128-taxon Yule tree with
+Exp(mean 0.05) branch lengths, 100,000 sites, exchangeabilities ~
+Dirichlet(1x6), base frequencies ~ Dirichlet(5x4), gamma shape 0.5 with 4
+equal-weight categories, 5% of cells gap/'N'.  Partials stored in FP32 with
+power-of-two scaling; arithmetic in FP64.
+
+A step is one full likelihood evaluation (phylo_loglik: the pruning launch
+over every internal node and site, the compensated total, and the 8-byte
+copy of the total back to the host) with all inputs resident on the GPU.
+One update = one (site, internal node, rate category) partial:
+(n-1) * m * C = 127 * 100000 * 4 = 5.08e7 per step.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Bytes per update the pruning pass must move, FP32 partials, one int32
+# scaler per (node, site) shared by the categories, 1-byte tip codes
+# (per site: C*S*4*(n-2) child reads + n tip bytes + C*S*4*(n-1) parent
+# writes + 4*(n-1) scaler writes + 4*(n-2) scaler reads, plus ~72 B root
+# reduction, divided by the (n-1)*C updates per site).
+def alg_bytes_per_update(n, S=4, C=4):
+    per_site = C * S * 4 * (n - 2) + n + C * S * 4 * (n - 1) + 4 * (n - 1) + 4 * (n - 2) + 72
+    return per_site / ((n - 1) * C)
+
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(wl, budget_s=20.0):
+    """Reference CPU path timed on this host: the compiled phylommand simpleML
+    (oracle/_ref/ref_harness, built with -O2 from the reference sources) on a
+    bounded slice of the same workload; falls back to the oracle's restatement
+    of the same cost model (P rebuilt per site and edge by squaring)."""
+    from oracle import oracle as O  # checker only: the CPU baseline leg
+    from phylommand_amd import synth
+    t = wl.tree
+    n = t.n_tips
+    harness = O.REF_HARNESS
+    m_sample = 400
+    cores = max(1, min(16, os.cpu_count() or 1))
+    if os.path.exists(harness) and os.access(harness, os.X_OK):
+        with tempfile.TemporaryDirectory() as d:
+            open(os.path.join(d, "t.nwk"), "w").write(t.newick() + "\n")
+            cmds = []
+            for k in range(cores):
+                # independent site shards: the reference is single-threaded and sites are independent
+                masks = wl.masks[:, k * m_sample:(k + 1) * m_sample]
+                fa = os.path.join(d, f"a{k}.fst")
+                open(fa, "w").write(synth.fasta_text(t.names, masks, synth.DNA_LETTERS, 4))
+                # the reference's sub_model cannot hold 6 distinct rates (heap overflow at
+                # sub_model.cpp:87), so the timing runs a 1-rate reversible model: the cost
+                # of a site-edge does not depend on the rate values.
+                cmds.append([harness, "time", os.path.join(d, "t.nwk"), fa, "dna", "1", "0,0,0,0,0,0", "1.0",
+                             ",".join("%.6f" % x for x in wl.pi[:3]), "3"])
+
+            def secs(out):
+                kv = dict(line.split(" ", 1) for line in out.strip().split("\n") if " " in line)
+                return float(kv["seconds"])
+
+            t0 = time.time()
+            one = secs(subprocess.run(cmds[0], capture_output=True, text=True, timeout=600).stdout)
+            procs = [subprocess.Popen(c, stdout=subprocess.PIPE, text=True) for c in cmds]
+            many = [secs(p.communicate(timeout=600)[0]) for p in procs]
+            wall = time.time() - t0
+            updates = (n - 1) * m_sample
+            rate_one = updates / one
+            rate_all = sum(updates / s for s in many)
+            return {"value": rate_all, "unit": "updates/s", "cores": cores, "kind": "reference",
+                    "single_core": rate_one,
+                    "sample": f"phylommand simpleML::calculate_log_likelihood compiled -O2 from the reference "
+                              f"sources, {n} taxa x {m_sample} sites per core, {cores} concurrent site shards "
+                              f"(1 core: {rate_one:.3g} updates/s); 1 rate category, the reference has no gamma "
+                              f"and its per-category cost is this; best of 3 calls per shard; {wall:.1f}s wall"}
+    masks = wl.masks[:, :m_sample]
+    # restatement of the reference's cost model
+    mb = np.zeros((t.n_nodes, m_sample), np.uint32)
+    mb[:n] = masks
+    p = O.Problem(t.left, t.right, t.root, t.blen, mb, 4, wl.Q, wl.pi, pi=wl.pi, builder=O.REFCOMPAT)
+    _, sec = p.time_reference_cost()
+    return {"value": (n - 1) * m_sample / sec, "unit": "updates/s", "cores": 1, "kind": "port",
+            "sample": f"oracle restatement of simpleML's cost model, {n} taxa x {m_sample} sites, 1 category"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--sites", type=int, default=100_000)
+    ap.add_argument("--taxa", type=int, default=128)
+    ap.add_argument("--fp64", action="store_true", help="store partials in FP64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              f"(sites shard across GPUs with one 8-byte all-reduce per evaluation; not built here)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import phylommand_amd as P
+    from phylommand_amd import synth
+
+    t0 = time.time()
+    wl = synth.gtr_gamma_workload(n_tips=args.taxa, m=args.sites, mean_blen=0.05, alpha=0.5, C=4,
+                                  gap_frac=0.05, seed=2)
+    gen_s = time.time() - t0
+    print(f"[bench] workload generated in {gen_s:.1f}s", file=sys.stderr, flush=True)
+    t = wl.tree
+    ctx = P.Context(0, P.PREC_FP64 if args.fp64 else P.PREC_FP32)
+    ctx.set_tips(wl.masks, 4)
+    ctx.set_topology(t.left, t.right, t.root)
+    ctx.set_edge_lengths(t.blen)
+    ctx.set_model(wl.Q, wl.pi, wl.pi, P.P_EIGEN)
+    ctx.set_categories(wl.cat_rates, wl.cat_weights)
+    lnl0 = ctx.loglik()
+    for _ in range(args.warmup):
+        ctx.loglik()
+    ctx.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        lnl = ctx.loglik()
+        kms.append(ctx.last_kernel_ms())
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    assert np.isfinite(lnl) and lnl == lnl0, (lnl, lnl0)
+
+    n, m, C = t.n_tips, args.sites, 4
+    updates = (n - 1) * m * C
+    value = updates * args.steps / elapsed
+    kernel_s = float(np.mean(kms)) * 1e-3
+    bpu = alg_bytes_per_update(n, 4, C) * (2 if args.fp64 else 1)
+    achieved = bpu * updates / kernel_s / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "prune4_gtr_g4_128x100k_traffic.json")
+    if os.path.exists(tf) and not args.fp64 and m == 100_000 and n == 128:
+        try:
+            traffic = json.load(open(tf)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    out = {
+        "metric": "CLV updates/s (site*node*cat)",
+        "value": value,
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Yule tree, GTR+G4 simulation)",
+        "config": {"workload": f"GTR+G4 DNA, {n} taxa Yule, {m} sites, 5% gaps",
+                   "n_taxa": n, "n_sites": m, "n_cat": C, "clv_storage": "f64" if args.fp64 else "f32",
+                   "arithmetic": "f64", "lnL": lnl},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "prune4_kernel<float,4>", "kernel_ms": kernel_s * 1e3,
+                     "alg_bytes_per_update": bpu},
+    }
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline(wl)
+        except Exception as e:  # the GPU figure stands on its own
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

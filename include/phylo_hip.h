@@ -111,6 +111,9 @@ int phylo_node_partials(phylo_ctx *ctx, int node, double *clv, int32_t *scale);
  * Replaces: sub_model::get_P_value after set_P_matrix (sub_model.h:46-50). */
 int phylo_get_pmatrix(phylo_ctx *ctx, int node, double *P);
 
+/* Waits for all work on the context's device (hipDeviceSynchronize). */
+int phylo_synchronize(phylo_ctx *ctx);
+
 /* Device time (ms, HIP events on the library's stream) of the pruning
  * kernel in the last phylo_loglik / phylo_nni_score call. */
 int phylo_last_kernel_ms(phylo_ctx *ctx, double *ms);

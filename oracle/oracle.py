@@ -45,6 +45,7 @@ def lib():
         L.orc_tree_newick.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
         L.orc_parse_fasta.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
                                       _up, ctypes.c_int, _ip, _ip]
+        L.orc_set_custom_alphabet.argtypes = [_up]
         L.orc_alphabet_mask.restype = ctypes.c_uint32
         L.orc_alphabet_mask.argtypes = [ctypes.c_int, ctypes.c_char]
         L.orc_n_states.argtypes = [_up, ctypes.c_long]
@@ -110,9 +111,32 @@ class OTree:
         return [v for v in range(self.n_nodes) if self.left[v] < 0 and self.right[v] < 0]
 
 
-def parse_fasta(text: str, alphabet: int = 0, set_empty: bool = True):
-    """Returns (names, masks[n_taxa, n_sites])."""
+def alphabet_table(text: str) -> np.ndarray:
+    """Character -> state-mask table of a whitespace alphabet file: each line is
+    a character followed by the state numbers it stands for
+    (alphabet_parser::pars_whitespace, matrix_parser.cpp:227-272)."""
+    table = np.zeros(256, np.uint32)
+    for line in text.splitlines():
+        parts = line.split()
+        if not parts:
+            continue
+        ch = line.lstrip()[0]
+        m = 0
+        for tok in parts[1:]:
+            if tok.isdigit() and int(tok) < 24:
+                m |= 1 << int(tok)
+        table[ord(ch)] = m
+    return table
+
+
+def parse_fasta(text: str, alphabet=0, set_empty: bool = True):
+    """Returns (names, masks[n_taxa, n_sites]).  alphabet: 0 dna, 1 protein,
+    2 binary, or a 256-entry table from alphabet_table()."""
     L = lib()
+    if not isinstance(alphabet, int):
+        tab = np.ascontiguousarray(alphabet, dtype=np.uint32)
+        L.orc_set_custom_alphabet(tab.ctypes.data_as(_up))
+        alphabet = 3
     nsites = ctypes.c_int(0)
     nt = L.orc_parse_fasta(text.encode(), alphabet, int(set_empty), None, 0, None, 0, None, ctypes.byref(nsites))
     names = ctypes.create_string_buffer(len(text) + 16)

@@ -250,12 +250,19 @@ static uint32_t bin_mask(char c) {
     return 0u;
 }
 
-/* alphabet: 0 dna, 1 protein, 2 binary.  Returns the all-states mask
- * (treeator.cpp:424-426). */
+/* A whitespace alphabet file (alphabet_parser::pars_whitespace,
+ * matrix_parser.cpp:227-272) as a 256-entry character table. */
+static uint32_t custom_table[256];
+void orc_set_custom_alphabet(const uint32_t *table) { memcpy(custom_table, table, sizeof custom_table); }
+
+/* alphabet: 0 dna, 1 protein, 2 binary, 3 custom table. */
 uint32_t orc_alphabet_mask(int alphabet, char c) {
+    if (alphabet == 3) return custom_table[(unsigned char)c];
     return alphabet == 1 ? aa_mask(c) : alphabet == 2 ? bin_mask(c) : dna_mask(c);
 }
+/* Union of every letter's states (treeator.cpp:424-426). */
 uint32_t orc_alphabet_all(int alphabet) {
+    if (alphabet == 3) { uint32_t a = 0; for (int i = 0; i < 256; ++i) a |= custom_table[i]; return a; }
     return alphabet == 1 ? ((1u << 23) - 1u) : alphabet == 2 ? 3u : 15u;
 }
 

@@ -42,6 +42,7 @@ SIGNATURES = [
     ("phylo_loglik", _c_int, [_c_void_p, _c_dbl_p, _c_dbl_p]),
     ("phylo_node_partials", _c_int, [_c_void_p, _c_int, _c_dbl_p, _c_int_p]),
     ("phylo_get_pmatrix", _c_int, [_c_void_p, _c_int, _c_dbl_p]),
+    ("phylo_synchronize", _c_int, [_c_void_p]),
     ("phylo_last_kernel_ms", _c_int, [_c_void_p, _c_dbl_p]),
 ]
 
@@ -173,6 +174,9 @@ class Context:
         P = np.zeros((n_cat, n_states, n_states))
         self._check(self.lib.phylo_get_pmatrix(self.h, int(node), _dp(P)))
         return P
+
+    def synchronize(self):
+        self._check(self.lib.phylo_synchronize(self.h))
 
     def last_kernel_ms(self) -> float:
         ms = np.zeros(1)

@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -42,6 +43,7 @@ constexpr int kBlock = 256;
 constexpr int kMaxCat = 8;
 constexpr int kMaxStates = 32;
 constexpr double kLn2 = 0.69314718055994530942;
+constexpr int kRootCatOff = 32;   // roots[]: per-state weights [0, 32), category weights after
 
 enum SrcKind : int { SRC_TIP = 0, SRC_CLV = 1, SRC_CARRY_A = 2, SRC_CARRY_B = 3, SRC_NONE = 4 };
 enum OpFlags : int { OP_ROOT = 1, OP_TO_B = 2 };
@@ -67,6 +69,11 @@ __device__ inline void load4(const double* p, double v[4]) {
     const double2 b = reinterpret_cast<const double2*>(p)[1];
     v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
 }
+__device__ inline void load_raw4(const float* p, float v[4]) {
+    const float4 x = *reinterpret_cast<const float4*>(p);
+    v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+}
+__device__ inline void load_raw4(const double* p, double v[4]) { load4(p, v); }
 __device__ inline void store4(float* p, const float v[4]) {
     *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
 }
@@ -75,171 +82,197 @@ __device__ inline void store4(double* p, const double v[4]) {
     reinterpret_cast<double2*>(p)[1] = make_double2(v[2], v[3]);
 }
 
-template <typename T, int C>
-__device__ inline void child_partial(int kind, int idx, int node, int s, int m_pad,
-                                     const uint8_t* __restrict__ tips, const T* clv, const int* scl,
-                                     const double* __restrict__ P, const T (&cA)[C][4], int sA,
-                                     const T (&cB)[C][4], int sB, double (&out)[C][4], int& sc) {
-    const double* Pn = P + (size_t)node * C * 16;
-    if (kind == SRC_TIP) {
-        const unsigned code = tips[(size_t)idx * m_pad + s];
-        double b[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) b[j] = (code >> j) & 1u ? 1.0 : 0.0;
-#pragma unroll
-        for (int c = 0; c < C; ++c)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                double a = 0.0;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) a = fma(Pn[c * 16 + i * 4 + j], b[j], a);
-                out[c][i] = a;
-            }
-        return;
-    }
-    double L[C][4];
-    if (kind == SRC_CLV) {
-#pragma unroll
-        for (int c = 0; c < C; ++c) load4(clv + (((size_t)idx * C + c) * m_pad + s) * 4, L[c]);
-        sc += scl[(size_t)idx * m_pad + s];
-    } else if (kind == SRC_CARRY_A) {
-#pragma unroll
-        for (int c = 0; c < C; ++c)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) L[c][i] = (double)cA[c][i];
-        sc += sA;
-    } else {
-#pragma unroll
-        for (int c = 0; c < C; ++c)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) L[c][i] = (double)cB[c][i];
-        sc += sB;
-    }
-#pragma unroll
-    for (int c = 0; c < C; ++c)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            double a = 0.0;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) a = fma(Pn[c * 16 + i * 4 + j], L[c][j], a);
-            out[c][i] = a;
-        }
-}
-
 __device__ inline double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
 
+// Sums v over the workgroup (any multiple of 64 threads); thread 0 stores it.
 __device__ inline void block_sum_store(double v, double* dst) {
-    __shared__ double red[kBlock / 64];
+    __shared__ double red[16];
     v = wave_sum(v);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) red[wid] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         double t = 0.0;
-#pragma unroll
-        for (int w = 0; w < kBlock / 64; ++w) t += red[w];
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
         *dst = t;
     }
 }
 
 // The pruning walker for S <= 4 (DNA, binary; padded states carry zero rows/cols of P).
-// grid = (m_pad / kBlock, n_programs); program y runs ops[prog_begin[y] .. prog_begin[y+1]).
+//
+// A workgroup owns 64 consecutive sites; wave c of the workgroup computes rate
+// category c for them, so the P matrices a wave needs are uniform across it
+// (scalar loads) and each lane holds one float4 partial.  The categories of a
+// site share one power-of-two scaler: each wave proposes the frexp exponent of
+// its per-site maximum, the proposals meet in LDS (double-buffered, one
+// barrier per op) and every wave applies the largest.
+// grid = (m_pad / 64, n_programs); program y runs ops[prog_begin[y] .. prog_begin[y+1]).
 template <typename T, int C>
-__global__ __launch_bounds__(kBlock) void prune4_kernel(
+__global__ __launch_bounds__(64 * C) void prune4_kernel(
     const Op* __restrict__ ops, const int* __restrict__ prog_begin, const uint8_t* __restrict__ tips,
     T* clv, int* scl, const double* __restrict__ P, const double* __restrict__ roots,
     const double* __restrict__ site_w, double* __restrict__ site_lnl, double* __restrict__ partial,
     int m, int m_pad) {
-    const int s = blockIdx.x * kBlock + threadIdx.x;
+    __shared__ int xexp[2][C][64];
+    __shared__ double xroot[C][64];
+    const int lane = threadIdx.x & 63;
+    const int cat = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int s = blockIdx.x * 64 + lane;
     const int prog = blockIdx.y;
     const int k0 = prog_begin[prog], k1 = prog_begin[prog + 1];
-    T cA[C][4], cB[C][4];
+    T cA[4] = {T(0), T(0), T(0), T(0)}, cB[4] = {T(0), T(0), T(0), T(0)};
     int sA = 0, sB = 0;
-#pragma unroll
-    for (int c = 0; c < C; ++c)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) { cA[c][i] = T(0); cB[c][i] = T(0); }
     double lnl = 0.0;
+    const T* plane = clv + ((size_t)cat * m_pad + s) * 4;          // + slot * C * m_pad * 4
+    const size_t slot_stride = (size_t)C * m_pad * 4;
     for (int k = k0; k < k1; ++k) {
         const Op op = ops[k];
-        double vl[C][4], vr[C][4];
+        const double* Pl = P + ((size_t)op.l_node * C + cat) * 16;
+        const double* Pr = P + ((size_t)op.r_node * C + cat) * 16;
+        // child inputs
+        double L[4], R[4];
         int sc = 0;
-        child_partial<T, C>(op.l_kind, op.l_idx, op.l_node, s, m_pad, tips, clv, scl, P, cA, sA, cB, sB, vl, sc);
-        if (op.r_kind != SRC_NONE) {
-            child_partial<T, C>(op.r_kind, op.r_idx, op.r_node, s, m_pad, tips, clv, scl, P, cA, sA, cB, sB, vr, sc);
+        {
+            double x[4];
+            if (op.l_kind == SRC_TIP) {
+                const unsigned code = tips[(size_t)op.l_idx * m_pad + s];
 #pragma unroll
-            for (int c = 0; c < C; ++c)
+                for (int j = 0; j < 4; ++j) x[j] = (code >> j) & 1u ? 1.0 : 0.0;
+            } else if (op.l_kind == SRC_CLV) {
+                load4(plane + op.l_idx * slot_stride, x);
+                sc += scl[(size_t)op.l_idx * m_pad + s];
+            } else if (op.l_kind == SRC_CARRY_A) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i) vl[c][i] *= vr[c][i];
-        }
-        double mx = 0.0;
+                for (int j = 0; j < 4; ++j) x[j] = (double)cA[j];
+                sc += sA;
+            } else {
 #pragma unroll
-        for (int c = 0; c < C; ++c)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) mx = fmax(mx, vl[c][i]);
-        int e = 0;
-        if (mx > 0.0) (void)frexp(mx, &e);
-        sc += e;
-        T r[C][4];
-#pragma unroll
-        for (int c = 0; c < C; ++c)
+                for (int j = 0; j < 4; ++j) x[j] = (double)cB[j];
+                sc += sB;
+            }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                vl[c][i] = ldexp(vl[c][i], -e);
-                r[c][i] = (T)vl[c][i];
-            }
-        if (op.out_slot >= 0) {
+                double a = 0.0;
 #pragma unroll
-            for (int c = 0; c < C; ++c) store4(clv + (((size_t)op.out_slot * C + c) * m_pad + s) * 4, r[c]);
-            scl[(size_t)op.out_slot * m_pad + s] = sc;
+                for (int j = 0; j < 4; ++j) a = fma(Pl[i * 4 + j], x[j], a);
+                L[i] = a;
+            }
+        }
+        if (op.r_kind != SRC_NONE) {
+            double x[4];
+            if (op.r_kind == SRC_TIP) {
+                const unsigned code = tips[(size_t)op.r_idx * m_pad + s];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) x[j] = (code >> j) & 1u ? 1.0 : 0.0;
+            } else if (op.r_kind == SRC_CLV) {
+                load4(plane + op.r_idx * slot_stride, x);
+                sc += scl[(size_t)op.r_idx * m_pad + s];
+            } else if (op.r_kind == SRC_CARRY_A) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) x[j] = (double)cA[j];
+                sc += sA;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) x[j] = (double)cB[j];
+                sc += sB;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                double a = 0.0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) a = fma(Pr[i * 4 + j], x[j], a);
+                R[i] = a;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) L[i] *= R[i];
+        }
+        // shared power-of-two scaler across the categories of a site
+        const double mx = fmax(fmax(L[0], L[1]), fmax(L[2], L[3]));
+        int e = INT_MIN;
+        if (mx > 0.0) (void)frexp(mx, &e);
+        if (C > 1) {
+            const int buf = k & 1;
+            xexp[buf][cat][lane] = e;
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < C; ++c) e = max(e, xexp[buf][c][lane]);
+        }
+        if (e == INT_MIN) e = 0;
+        sc += e;
+        T r[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            L[i] = ldexp(L[i], -e);
+            r[i] = (T)L[i];
+        }
+        if (op.out_slot >= 0) {
+            store4(const_cast<T*>(plane) + op.out_slot * slot_stride, r);
+            if (cat == 0) scl[(size_t)op.out_slot * m_pad + s] = sc;
         }
         if (op.flags & OP_TO_B) {
 #pragma unroll
-            for (int c = 0; c < C; ++c)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) cB[c][i] = r[c][i];
+            for (int i = 0; i < 4; ++i) cB[i] = r[i];
             sB = sc;
         } else {
 #pragma unroll
-            for (int c = 0; c < C; ++c)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) cA[c][i] = r[c][i];
+            for (int i = 0; i < 4; ++i) cA[i] = r[i];
             sA = sc;
         }
         if (op.flags & OP_ROOT) {
-            double like = 0.0;
+            double lc = 0.0;
 #pragma unroll
-            for (int c = 0; c < C; ++c) {
-                double lc = 0.0;
+            for (int i = 0; i < 4; ++i) lc = fma(roots[i], L[i], lc);
+            lc *= roots[kRootCatOff + cat];
+            if (C > 1) {
+                xroot[cat][lane] = lc;
+                __syncthreads();
+                lc = 0.0;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) lc = fma(roots[i], vl[c][i], lc);
-                like = fma(roots[4 + c], lc, like);
+                for (int c = 0; c < C; ++c) lc += xroot[c][lane];
+                __syncthreads();
             }
-            lnl = log(like) + sc * kLn2;
+            lnl = log(lc) + sc * kLn2;
         }
     }
-    const bool live = s < m;
+    const bool live = s < m && cat == 0;
     if (live && site_lnl) site_lnl[(size_t)prog * m + s] = lnl;
     const double w = live ? (site_w ? site_w[s] : 1.0) : 0.0;
     block_sum_store(live ? w * lnl : 0.0, partial + (size_t)prog * gridDim.x + blockIdx.x);
 }
 
 // Compensated fixed-order sum of the per-workgroup partials of each program.
-__global__ void final_sum_kernel(const double* __restrict__ partial, int n_part, int n_prog, double* __restrict__ out) {
-    const int y = blockIdx.x * blockDim.x + threadIdx.x;
-    if (y >= n_prog) return;
+// Compensated fixed-order sum of the per-workgroup partials of each program
+// (one 256-thread block per program: strided Kahan sums, then a fixed tree).
+__global__ __launch_bounds__(256) void final_sum_kernel(const double* __restrict__ partial, int n_part,
+                                                       double* __restrict__ out) {
+    __shared__ double hi[256], lo[256];
+    const int y = blockIdx.x;
     double s = 0.0, comp = 0.0;
-    for (int i = 0; i < n_part; ++i) {
+    for (int i = threadIdx.x; i < n_part; i += 256) {
         const double x = partial[(size_t)y * n_part + i] - comp;
         const double t = s + x;
         comp = (t - s) - x;
         s = t;
     }
-    out[y] = s;
+    hi[threadIdx.x] = s;
+    lo[threadIdx.x] = -comp;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            const double a = hi[threadIdx.x], b = hi[threadIdx.x + w];
+            const double t = a + b;
+            const double bb = t - a;
+            const double err = (a - (t - bb)) + (b - bb);   // two-sum
+            hi[threadIdx.x] = t;
+            lo[threadIdx.x] += lo[threadIdx.x + w] + err;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[y] = hi[0] + lo[0];
 }
 
 // P[node][cat][i][j] = sum_k V[i][k] exp(lam_k r_c t_node) Vinv[k][j], padded to Sp x Sp with zeros.
@@ -293,6 +326,7 @@ struct phylo_ctx {
     int C = 1;
     std::vector<double> cat_rate{1.0}, cat_w{1.0};
     bool dirty_P = true;
+    bool dirty_roots = true;
 
     // compiled program
     std::vector<Op> ops;
@@ -484,18 +518,21 @@ int prepare(phylo_ctx* c) {
         int rc = upload_pmatrices(c);
         if (rc) return rc;
     }
-    std::vector<double> roots(4 + kMaxCat, 0.0);
-    for (int i = 0; i < c->S; ++i) roots[i] = c->root_w[i];
-    for (int k = 0; k < c->C; ++k) roots[4 + k] = c->cat_w[k];
-    if (!c->d_roots) HIPCHK(c, hipMalloc(&c->d_roots, roots.size() * sizeof(double)));
-    HIPCHK(c, hipMemcpyAsync(c->d_roots, roots.data(), roots.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->dirty_roots) {
+        std::vector<double> roots(kMaxStates + kMaxCat, 0.0);
+        for (int i = 0; i < c->S; ++i) roots[i] = c->root_w[i];
+        for (int k = 0; k < c->C; ++k) roots[kRootCatOff + k] = c->cat_w[k];
+        if (!c->d_roots) HIPCHK(c, hipMalloc(&c->d_roots, roots.size() * sizeof(double)));
+        HIPCHK(c, hipMemcpyAsync(c->d_roots, roots.data(), roots.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->dirty_roots = false;
+    }
     return PHYLO_OK;
 }
 
 template <typename T, int C>
 void launch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
-    hipLaunchKernelGGL((prune4_kernel<T, C>), grid, dim3(kBlock), 0, c->stream, c->d_ops, c->d_prog, c->d_tips,
+    hipLaunchKernelGGL((prune4_kernel<T, C>), grid, dim3(64 * C), 0, c->stream, c->d_ops, c->d_prog, c->d_tips,
                        reinterpret_cast<T*>(c->d_clv), c->d_scl, c->d_P, c->d_roots,
                        c->have_site_w ? c->d_sitew : nullptr, site_lnl, c->d_partial, c->m, c->m_pad);
 }
@@ -515,7 +552,7 @@ int dispatch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
 // Launches the program set currently in c->ops / c->prog_begin and returns
 // one total per program in totals (n_prog values); site_lnl [n_prog][m] is optional.
 int run_programs(phylo_ctx* c, int n_prog, double* totals, double* site_lnl_host) {
-    const int nblk = c->m_pad / kBlock;
+    const int nblk = c->m_pad / 64;
     HIPCHK(c, ensure(c->d_partial, c->partial_bytes, (size_t)n_prog * nblk * sizeof(double)));
     HIPCHK(c, ensure(c->d_total, c->total_bytes, (size_t)n_prog * sizeof(double)));
     if (site_lnl_host) HIPCHK(c, ensure(c->d_sitelnl, c->sitelnl_bytes, (size_t)n_prog * c->m * sizeof(double)));
@@ -532,7 +569,7 @@ int run_programs(phylo_ctx* c, int n_prog, double* totals, double* site_lnl_host
     if (rc) return rc;
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    hipLaunchKernelGGL(final_sum_kernel, dim3((n_prog + 63) / 64), dim3(64), 0, c->stream, c->d_partial, nblk, n_prog, c->d_total);
+    hipLaunchKernelGGL(final_sum_kernel, dim3(n_prog), dim3(256), 0, c->stream, c->d_partial, nblk, c->d_total);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(c->h_total, c->d_total, (size_t)n_prog * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     if (site_lnl_host)
@@ -619,6 +656,7 @@ int phylo_set_tips(phylo_ctx* c, int n_taxa, int n_sites, int n_states, const ui
     for (auto& x : c->masks) x &= valid;
     c->have_tips = true;
     c->dirty_tips = true;
+    c->dirty_roots = true;
     c->dirty_topo = true;     // slot numbering depends on n_taxa
     c->have_site_w = false;
     c->dirty_site_w = false;
@@ -701,6 +739,7 @@ int phylo_set_model(phylo_ctx* c, int n_states, const double* Q, const double* r
     }
     c->have_model = true;
     c->dirty_P = true;
+    c->dirty_roots = true;
     return PHYLO_OK;
 }
 
@@ -714,6 +753,7 @@ int phylo_set_categories(phylo_ctx* c, int n_cat, const double* rates, const dou
     c->cat_rate.assign(rates, rates + n_cat);
     c->cat_w.assign(weights, weights + n_cat);
     c->dirty_P = true;
+    c->dirty_roots = true;
     if (c->d_clv) { (void)hipFree(c->d_clv); c->d_clv = nullptr; c->clv_bytes = 0; }
     return PHYLO_OK;
 }
@@ -772,6 +812,13 @@ int phylo_get_pmatrix(phylo_ctx* c, int node, double* P) {
     for (int k = 0; k < c->C; ++k)
         for (int i = 0; i < S; ++i)
             for (int j = 0; j < S; ++j) P[((size_t)k * S + i) * S + j] = buf[((size_t)k * Sp + i) * Sp + j];
+    return PHYLO_OK;
+}
+
+int phylo_synchronize(phylo_ctx* c) {
+    if (!c) return PHYLO_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipDeviceSynchronize());
     return PHYLO_OK;
 }
 

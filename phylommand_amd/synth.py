@@ -223,6 +223,9 @@ def fasta_text(names, masks, letters: str, S: int) -> str:
     (DNA 'N' also maps to all states)."""
     full = (1 << S) - 1
     lut = {1 << i: letters[i] for i in range(S)}
+    if letters == DNA_LETTERS:
+        # IUPAC codes in the A,G,C,T bit order of alphabet::set_alphabet_dna
+        lut.update({3: "R", 12: "Y", 6: "S", 9: "W", 10: "K", 5: "M", 14: "B", 11: "D", 13: "H", 7: "V"})
     lut[full] = "-"
     out = []
     for name, row in zip(names, masks):

@@ -1,0 +1,261 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures under tests/golden/ by running the compiled
+reference (oracle/_ref/ref_harness, built by `make -C oracle ref` from the
+phylommand sources) on seeded inputs.  Run in a container where the reference
+sources exist; the fixtures (inputs + expected outputs) are committed.
+
+    python tests/golden/make_golden.py
+"""
+import json
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from phylommand_amd import synth  # noqa: E402
+
+H = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+REF_EXAMPLES = "/root/reference/example_files"
+
+
+def run(*args):
+    out = subprocess.run([H, *map(str, args)], capture_output=True, text=True, timeout=600)
+    if out.returncode != 0:
+        raise RuntimeError(f"{args}: rc={out.returncode} {out.stderr[-500:]}")
+    return out.stdout
+
+
+def write(path, text):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        f.write(text)
+
+
+def persite_case(name, tree_text, fasta_text, alpha, tr, spec, params, freqs, meta):
+    d = os.path.join(HERE, name)
+    write(os.path.join(d, "tree.nwk"), tree_text)
+    write(os.path.join(d, "aln.fst"), fasta_text)
+    alpha_arg = alpha
+    if alpha not in ("dna", "protein", "binary"):
+        shutil.copy(alpha, os.path.join(d, "alphabet.txt"))
+        alpha_arg = os.path.join(d, "alphabet.txt")
+    out = run("persite", os.path.join(d, "tree.nwk"), os.path.join(d, "aln.fst"), alpha_arg, int(tr), spec, params, freqs)
+    write(os.path.join(d, "expected.txt"), out)
+    meta = dict(meta, alphabet=alpha if alpha in ("dna", "protein", "binary") else "alphabet.txt", tr=int(tr),
+                spec=spec, params=params, freqs=freqs)
+    write(os.path.join(d, "case.json"), json.dumps(meta, indent=1) + "\n")
+
+
+def main():
+    if not os.path.exists(H):
+        subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle"), "ref"])
+
+    # C1: 16-taxon Yule tree (seed 1), Exp(0.1) lengths, 1000 JC69 sites, no gaps.
+    # JC69 in substitution units is the reference's TR rate 4/3 with pi = 1/4
+    # (Q is unnormalised, sub_model.cpp:99-114).  The tree is wrapped as
+    # (T:1,DUMMY:1) with an all-'N' DUMMY so the reference's zeroed root edges
+    # (rate_model.cpp:106,122) leave the textbook likelihood of T.
+    rng = np.random.default_rng(1)
+    t = synth.yule_tree(16, rng, 0.1)
+    pi = np.full(4, 0.25)
+    Q = synth.gtr_Q(np.full(6, 4.0 / 3.0), pi)
+    st = synth.simulate(t, Q, pi, 1000, rng)
+    masks = synth.masks_from_states(st[: t.n_tips], 0.0, rng, 4)
+    tw = synth.wrap_with_dummy(t)
+    mw = np.vstack([masks, np.full((1, 1000), 15, np.uint32)])
+    persite_case("c1_jc16", tw.newick() + "\n", synth.fasta_text(tw.names, mw, synth.DNA_LETTERS, 4).replace("-", "N"),
+                 "dna", True, "0,0,0,0,0,0", "1.3333333333333333", "0.25,0.25,0.25,0.25",
+                 {"what": "C1: 16 taxa x 1000 JC69 sites, tree wrapped with an all-N DUMMY taxon"})
+
+    # Survey probe cases on the 4-taxon tree ((A:0.1,B:0.2):0.05,(C:0.1,D:0.3):0.05), data A,A,G,T
+    t4 = "((A:0.1,B:0.2):0.05,(C:0.1,D:0.3):0.05);\n"
+    t4w = "(((A:0.1,B:0.2):0.05,(C:0.1,D:0.3):0.05):1.0,DUMMY:1.0);\n"
+    t4u = "(A:0.1,B:0.2,(C:0.1,D:0.3):0.05);\n"
+    d4 = ">A\nA\n>B\nA\n>C\nG\n>D\nT\n"
+    d4w = d4 + ">DUMMY\nN\n"
+    persite_case("t4_jc_rootzero", t4, d4, "dna", True, "0,0,0,0,0,0", "1.3333333333333333", "0.25,0.25,0.25,0.25",
+                 {"what": "root children zeroed by the reference's rate model"})
+    persite_case("t4_jc_wrapped", t4w, d4w, "dna", True, "0,0,0,0,0,0", "1.3333333333333333", "0.25,0.25,0.25,0.25",
+                 {"what": "textbook JC69 through the DUMMY wrapper"})
+    persite_case("t4_nontr", t4w, d4w, "dna", False, "0,0,0,0,0,0,0,0,0,0,0,0", "0.3333333333333333", "-",
+                 {"what": "non-reversible model: root sum unweighted (simpleML.h:133)"})
+    persite_case("t4_2param", t4w, d4w, "dna", True, "0,1,0,0,1,0", "1.0,0.25", "0.1,0.2,0.3,0.4",
+                 {"what": "2-rate reversible model with uneven frequencies"})
+    persite_case("t4_unrooted", t4u, d4, "dna", True, "0,0,0,0,0,0", "1.3333333333333333", "0.25,0.25,0.25,0.25",
+                 {"what": "trifurcating root resolved with a zero-length node (tree.cpp:146-158)"})
+
+    # 3-rate DNA model, 24 taxa x 300 sites with 5% gaps and IUPAC ambiguity codes
+    rng = np.random.default_rng(7)
+    t = synth.yule_tree(24, rng, 0.08)
+    pi = np.array([0.3, 0.2, 0.15, 0.35])
+    rates = np.array([1.0, 2.5, 0.7, 0.7, 2.5, 1.0])
+    Q = synth.gtr_Q(rates, pi)
+    st = synth.simulate(t, Q, pi, 300, rng)
+    masks = synth.masks_from_states(st[: t.n_tips], 0.05, rng, 4)
+    amb = rng.random(masks.shape) < 0.02
+    masks[amb] = np.uint32(1 | 4)   # 'M' = A or C
+    tw = synth.wrap_with_dummy(t)
+    mw = np.vstack([masks, np.full((1, 300), 15, np.uint32)])
+    fa = synth.fasta_text(tw.names, mw, synth.DNA_LETTERS, 4)
+    lines = fa.split("\n")
+    for i in range(1, len(lines), 2):
+        if lines[i]:
+            row = mw[i // 2]
+            lines[i] = "".join("M" if x == 5 else c for c, x in zip(lines[i], row))
+    fa = "\n".join(lines)
+    persite_case("dna3_24x300", tw.newick() + "\n", fa, "dna", True, "0,1,2,2,1,0", "1.0,2.5,0.7",
+                 "0.3,0.2,0.15,0.35", {"what": "3-rate reversible DNA, gaps and ambiguity codes"})
+
+    # non-reversible DNA with 3 distinct rates, 10 taxa x 200 sites
+    rng = np.random.default_rng(11)
+    t = synth.yule_tree(10, rng, 0.1)
+    pi = np.full(4, 0.25)
+    st = synth.simulate(t, synth.gtr_Q(np.full(6, 4 / 3), pi), pi, 200, rng)
+    masks = synth.masks_from_states(st[: t.n_tips], 0.03, rng, 4)
+    tw = synth.wrap_with_dummy(t)
+    mw = np.vstack([masks, np.full((1, 200), 15, np.uint32)])
+    persite_case("nontr_dna_10x200", tw.newick() + "\n", synth.fasta_text(tw.names, mw, synth.DNA_LETTERS, 4), "dna",
+                 False, "0,1,2,1,0,2,2,0,1,1,2,0", "0.2,0.35,0.1", "-",
+                 {"what": "non-reversible 3-rate DNA model (default treeator model family)"})
+
+    # binary alphabet, non-reversible 2-rate model (0->1 and 1->0 differ)
+    rng = np.random.default_rng(13)
+    t = synth.yule_tree(8, rng, 0.2)
+    pi2 = np.array([0.5, 0.5])
+    st = synth.simulate(t, np.array([[-1.0, 1.0], [1.0, -1.0]]), pi2, 40, rng)
+    masks = synth.masks_from_states(st[: t.n_tips], 0.05, rng, 2)
+    tw = synth.wrap_with_dummy(t)
+    mw = np.vstack([masks, np.full((1, 40), 3, np.uint32)])
+    persite_case("binary_8x40", tw.newick() + "\n", synth.fasta_text(tw.names, mw, "01", 2), "binary", False, "0,1",
+                 "0.7,0.3", "-", {"what": "binary alphabet, non-reversible"})
+
+    # 20-state protein through a whitespace alphabet file (ARNDCQEGHILKMFPSTWYV = 0..19)
+    alpha_path = os.path.join(HERE, "aa20.alphabet.tmp")
+    with open(alpha_path, "w") as f:
+        for i, ch in enumerate(synth.AA20_LETTERS):
+            f.write(f"{ch} {i}\n")
+        f.write("- " + " ".join(str(i) for i in range(20)) + "\n")
+        f.write("X " + " ".join(str(i) for i in range(20)) + "\n")
+    rng = np.random.default_rng(17)
+    t = synth.yule_tree(8, rng, 0.15)
+    pi20 = rng.dirichlet(np.full(20, 2.0))
+    Q20 = synth.gtr_Q(np.full(190, 1.0), pi20)
+    st = synth.simulate(t, Q20, pi20, 60, rng)
+    masks = synth.masks_from_states(st[: t.n_tips], 0.03, rng, 20)
+    tw = synth.wrap_with_dummy(t)
+    mw = np.vstack([masks, np.full((1, 60), (1 << 20) - 1, np.uint32)])
+    persite_case("protein20_8x60", tw.newick() + "\n", synth.fasta_text(tw.names, mw, synth.AA20_LETTERS, 20),
+                 alpha_path, True, ",".join(["0"] * 190), "1.0", ",".join("%.17g" % x for x in pi20),
+                 {"what": "20-state reversible 1-rate protein model via an alphabet file"})
+    os.remove(alpha_path)
+
+    # the reference's own example files: first character (what the CLI scores,
+    # treeator.cpp:723-724) and every site, under a 1-rate and a 2-rate model
+    d = os.path.join(HERE, "example_files")
+    os.makedirs(d, exist_ok=True)
+    shutil.copy(os.path.join(REF_EXAMPLES, "tree_file.tree"), os.path.join(d, "tree_file.tree"))
+    shutil.copy(os.path.join(REF_EXAMPLES, "alignment_file.fst"), os.path.join(d, "alignment_file.fst"))
+    for nm, spec, par, fr in [("jc", "0,0,0,0,0,0", "1.3333333333333333", "0.25,0.25,0.25,0.25"),
+                              ("k2", "0,1,0,0,1,0", "1.0,0.25", "0.1,0.2,0.3,0.4")]:
+        out = run("persite", os.path.join(d, "tree_file.tree"), os.path.join(d, "alignment_file.fst"), "dna", 1, spec,
+                  par, fr)
+        write(os.path.join(d, f"expected_{nm}.txt"), out)
+    write(os.path.join(d, "case.json"), json.dumps({"what": "phylommand example_files, root edges zeroed",
+                                                    "models": {"jc": ["0,0,0,0,0,0", "1.3333333333333333",
+                                                                      "0.25,0.25,0.25,0.25"],
+                                                               "k2": ["0,1,0,0,1,0", "1.0,0.25",
+                                                                      "0.1,0.2,0.3,0.4"]}}, indent=1) + "\n")
+
+    # refcompat P(t) dumps (sub_model::set_P_matrix)
+    ts = [1e-6, 0.01, 0.1, 0.5, 1.0, 3.0]
+    pm = {}
+    for key, S, tr, spec, par, fr in [
+            ("jc4", 4, 1, "0,0,0,0,0,0", "1.3333333333333333", "0.25,0.25,0.25,0.25"),
+            ("k2_4", 4, 1, "0,1,0,0,1,0", "1.0,0.25", "0.1,0.2,0.3,0.4"),
+            ("r3_4", 4, 1, "0,1,2,2,1,0", "1.0,2.5,0.7", "0.3,0.2,0.15,0.35"),
+            ("nontr4", 4, 0, "0,1,2,1,0,2,2,0,1,1,2,0", "0.2,0.35,0.1", "-"),
+            ("aa20", 20, 1, ",".join(["0"] * 190), "1.0", ",".join("%.17g" % x for x in pi20))]:
+        out = run("pmatrix", S, tr, spec, par, fr, *ts)
+        vals = [[float(x) for x in line.split()] for line in out.strip().split("\n")]
+        pm[key] = {"S": S, "tr": tr, "spec": spec, "params": par, "freqs": fr, "t": ts,
+                   "P": [vals[i * S:(i + 1) * S] for i in range(len(ts))]}
+    write(os.path.join(HERE, "pmatrix_refcompat.json"), json.dumps(pm) + "\n")
+
+    # rate_model::get_time_mod_branch_length dumps
+    tm = []
+    pairs = [(0.1, 0.0), (0.1, 0.05), (0.3, 0.2), (0.5, 0.9), (2.0, 0.0), (0.25, 1.5), (0.7, 0.45), (1e-3, 2.9)]
+    for arg in ["-", "0.5:1.0", "2.0:0.3,0.25:1.0", "3.0:0.2,0.5:0.6,1.5:1.2"]:
+        flat = [x for p in pairs for x in p]
+        out = run("tmbl", arg, *flat)
+        tm.append({"arg": arg, "pairs": pairs, "out": [float(x) for x in out.split()]})
+    write(os.path.join(HERE, "time_mod_length.json"), json.dumps(tm) + "\n")
+
+    # whole-tree likelihoods with rate models (time cut-offs, clade rates)
+    rng = np.random.default_rng(19)
+    t = synth.yule_tree(8, rng, 0.15)
+    pi = np.array([0.25, 0.25, 0.25, 0.25])
+    st = synth.simulate(t, synth.gtr_Q(np.full(6, 4 / 3), pi), pi, 50, rng)
+    masks = synth.masks_from_states(st[: t.n_tips], 0.0, rng, 4)
+    d = os.path.join(HERE, "ratemodel_8x50")
+    write(os.path.join(d, "tree.nwk"), t.newick() + "\n")
+    write(os.path.join(d, "aln.fst"), synth.fasta_text(t.names, masks, synth.DNA_LETTERS, 4))
+    rl = []
+    # (two or more clade sets crash the reference with a segmentation fault, so
+    # each clade case names one set)
+    for targ, carg in [("-", "-"), ("0.5:0.1", "-"), ("2.0:0.05,0.5:0.2", "-"), ("-", "2.0:t0,t1"),
+                       ("-", "3.0:t2,t3,t4"), ("-", "0.5:t5,t6")]:
+        out = run("ratelik", os.path.join(d, "tree.nwk"), os.path.join(d, "aln.fst"), "dna", 1, "0,0,0,0,0,0",
+                  "1.3333333333333333", "0.25,0.25,0.25,0.25", targ, carg)
+        rl.append({"time_arg": targ, "clade_arg": carg, "total": float(out.split()[1])})
+    write(os.path.join(d, "expected.json"), json.dumps(rl, indent=1) + "\n")
+
+    # draw_normalized_likelihood_on_nodes labels, 3 sites
+    d = os.path.join(HERE, "nodes_6x3")
+    rng = np.random.default_rng(23)
+    t = synth.yule_tree(6, rng, 0.2)
+    st = synth.simulate(t, synth.gtr_Q(np.full(6, 4 / 3), pi), pi, 3, rng)
+    masks = synth.masks_from_states(st[: t.n_tips], 0.0, rng, 4)
+    tw = synth.wrap_with_dummy(t)
+    mw = np.vstack([masks, np.full((1, 3), 15, np.uint32)])
+    write(os.path.join(d, "tree.nwk"), tw.newick() + "\n")
+    write(os.path.join(d, "aln.fst"), synth.fasta_text(tw.names, mw, synth.DNA_LETTERS, 4).replace("-", "N"))
+    out = run("nodes", os.path.join(d, "tree.nwk"), os.path.join(d, "aln.fst"), "dna", 1, "0,1,0,0,1,0", "1.0,0.25",
+              "0.1,0.2,0.3,0.4", 3)
+    write(os.path.join(d, "expected.txt"), out)
+
+    # newick parsing: comments, quotes, polytomies, labels, whitespace, rooted on a leaf
+    d = os.path.join(HERE, "parse")
+    inputs = ["((A:0.1,B:0.2)90:0.05,(C:0.1,D:0.3)x:0.05);",
+              "(A:1,B:2,C:3,D:4);",
+              "((A,B,C),(D,E));",
+              "[a comment](('taxon one':0.5,\"t 2\":0.25)[&support=1]:0.1, C : 0.2 ) ;",
+              "(A:0.1,(B:0.2,(C:0.3,D:0.4)[[nested]]:0.5):0.6)root;",
+              "((A:1e-3,B:2.5E-2):0.0,C:7)lab;",
+              "((A:0.1,B:0.2):0.3)Z:0.4;"]
+    write(os.path.join(d, "input.nwk"), "\n".join(inputs) + "\n")
+    outs = []
+    for s in inputs:
+        p = os.path.join(d, "one.tmp")
+        write(p, s + "\n")
+        outs.append(run("parse", p).strip())
+        os.remove(p)
+    write(os.path.join(d, "expected.txt"), "\n".join(outs) + "\n")
+
+    # treebender --nni all lists
+    d = os.path.join(HERE, "nni")
+    for n in (5, 8, 16):
+        rng = np.random.default_rng(100 + n)
+        t = synth.yule_tree(n, rng, 0.1)
+        write(os.path.join(d, f"tree{n}.nwk"), t.newick(6) + "\n")
+        out = run("nni", os.path.join(d, f"tree{n}.nwk"))
+        write(os.path.join(d, f"expected{n}.txt"), out)
+    print("fixtures written under", HERE)
+
+
+if __name__ == "__main__":
+    main()

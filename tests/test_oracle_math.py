@@ -1,0 +1,243 @@
+"""Pins the oracle's model maths and pruning against closed forms, invariants,
+independent library routines (scipy) and brute force.  CPU only."""
+import itertools
+
+import numpy as np
+import pytest
+import scipy.linalg
+import scipy.special
+
+from phylommand_amd import synth
+
+
+def rand_gtr(rng, S=4):
+    pi = rng.dirichlet(np.full(S, 3.0))
+    r = rng.gamma(1.0, 1.0, S * (S - 1) // 2)
+    return synth.gtr_Q(r, pi), pi
+
+
+def test_jc69_closed_form(oracle):
+    Q = oracle.build_Q(4, 1, [0] * 6, [4 / 3], [0.25] * 3)
+    for t in [0.0, 1e-4, 0.05, 0.3, 2.0]:
+        want_d = 0.25 + 0.75 * np.exp(-4 * t / 3)
+        want_o = 0.25 - 0.25 * np.exp(-4 * t / 3)
+        for b in (oracle.EIG, oracle.PADE):
+            P = oracle.pmatrix(Q, t, b, pi=np.full(4, 0.25))
+            np.testing.assert_allclose(np.diag(P), want_d, rtol=1e-13, atol=1e-15)
+            np.testing.assert_allclose(P[0, 1:], want_o, rtol=1e-12, atol=1e-15)
+
+
+def test_k80_closed_form(oracle):
+    # K80 in the reference's A,G,C,T order: transitions A<->G (pos 0) and C<->T (pos 5)
+    kappa = 3.0
+    pi = np.full(4, 0.25)
+    Q = synth.gtr_Q([kappa, 1, 1, 1, 1, kappa], pi)
+    b = -np.dot(pi, np.diag(Q))
+    for t in [0.01, 0.2, 1.0]:
+        d = t * b
+        ek = np.exp(-4 * d / (kappa + 2))
+        ek2 = np.exp(-2 * d * (kappa + 1) / (kappa + 2))
+        p_same = 0.25 + 0.25 * ek + 0.5 * ek2
+        p_ts = 0.25 + 0.25 * ek - 0.5 * ek2
+        p_tv = 0.25 - 0.25 * ek
+        P = oracle.pmatrix(Q, t, oracle.EIG, pi=pi)
+        np.testing.assert_allclose([P[0, 0], P[0, 1], P[0, 2]], [p_same, p_ts, p_tv], rtol=1e-12)
+
+
+@pytest.mark.parametrize("S", [2, 4, 20])
+def test_pmatrix_against_scipy_and_invariants(oracle, S):
+    rng = np.random.default_rng(S)
+    Q, pi = rand_gtr(rng, S)
+    for t in [1e-3, 0.1, 0.7, 3.0]:
+        want = scipy.linalg.expm(Q * t)
+        for b in (oracle.EIG, oracle.PADE):
+            P = oracle.pmatrix(Q, t, b, pi=pi)
+            np.testing.assert_allclose(P, want, rtol=1e-9, atol=1e-13)
+            np.testing.assert_allclose(P.sum(1), 1.0, atol=1e-12)
+            # detailed balance
+            np.testing.assert_allclose(pi[:, None] * P, (pi[:, None] * P).T, atol=1e-13)
+        # the reference's squaring builder: same matrix to its ~1e-7 error
+        np.testing.assert_allclose(oracle.pmatrix(Q, t, oracle.REFCOMPAT), want, rtol=1e-5, atol=1e-9)
+    # Chapman-Kolmogorov P(s)P(t) = P(s+t), P(0) = I
+    Ps, Pt = oracle.pmatrix(Q, 0.2, oracle.EIG, pi), oracle.pmatrix(Q, 0.5, oracle.EIG, pi)
+    np.testing.assert_allclose(Ps @ Pt, oracle.pmatrix(Q, 0.7, oracle.EIG, pi), atol=1e-12)
+    np.testing.assert_allclose(oracle.pmatrix(Q, 0.0, oracle.EIG, pi), np.eye(S), atol=1e-14)
+
+
+def test_nonreversible_pade_against_scipy(oracle):
+    rng = np.random.default_rng(5)
+    Q = rng.gamma(1.0, 0.3, (4, 4))
+    np.fill_diagonal(Q, 0)
+    np.fill_diagonal(Q, -Q.sum(1))
+    for t in [0.05, 1.0, 10.0]:
+        np.testing.assert_allclose(oracle.pmatrix(Q, t, oracle.PADE), scipy.linalg.expm(Q * t), rtol=1e-10,
+                                   atol=1e-14)
+
+
+def test_rate_positions_match_reference_layout(oracle):
+    """get_rate_pos (sub_model.cpp:3-29): reversible slots are the upper
+    triangle in row-major order, non-reversible slots all off-diagonals."""
+    S = 4
+    got = [[oracle.lib().orc_rate_pos(S, 1, i, j) for j in range(S)] for i in range(S)]
+    want = [[6, 0, 1, 2], [0, 6, 3, 4], [1, 3, 6, 5], [2, 4, 5, 6]]
+    assert got == want
+    got = [[oracle.lib().orc_rate_pos(S, 0, i, j) for j in range(S)] for i in range(S)]
+    want = [[12, 0, 1, 2], [3, 12, 4, 5], [6, 7, 12, 8], [9, 10, 11, 12]]
+    assert got == want
+
+
+def test_freqs_last_is_remainder(oracle):
+    f = oracle.freqs(4, [0.1, 0.2, 0.3, 0.9])     # a given 4th value is ignored (sub_model.cpp:47-51)
+    np.testing.assert_allclose(f, [0.1, 0.2, 0.3, 0.4])
+    np.testing.assert_allclose(oracle.freqs(5), np.full(5, 0.2))
+
+
+def test_gamma_rates_against_scipy_and_yang(oracle):
+    for alpha in [0.1, 0.5, 1.0, 2.0, 10.0]:
+        for C in [2, 4, 8]:
+            r = oracle.gamma_rates(alpha, C)
+            q = scipy.special.gammaincinv(alpha, np.arange(1, C) / C) / alpha
+            I = np.concatenate([[0], scipy.special.gammainc(alpha + 1, q * alpha), [1]])
+            np.testing.assert_allclose(r, np.diff(I) * C, rtol=1e-9)
+            assert abs(r.mean() - 1.0) < 1e-12
+    # Yang (1994) Table: alpha = 0.5, K = 4 mean rates
+    np.testing.assert_allclose(oracle.gamma_rates(0.5, 4), [0.0334, 0.2519, 0.8203, 2.8944], atol=5e-5)
+    for a, x in [(0.5, 0.3), (2.0, 5.0), (7.5, 3.0), (1.0, 40.0)]:
+        assert abs(oracle.lib().orc_gammainc(a, x) - scipy.special.gammainc(a, x)) < 1e-13
+
+
+def _brute_force(tree_left, tree_right, root, eff, masks_by_node, Q, pi, S, C_rates, C_w, n_nodes):
+    """Sum over every assignment of states to internal nodes."""
+    internal = [v for v in range(n_nodes) if tree_left[v] >= 0]
+    tips = [v for v in range(n_nodes) if tree_left[v] < 0]
+    m = masks_by_node.shape[1]
+    out = np.zeros(m)
+    for s in range(m):
+        tot = 0.0
+        for r, w in zip(C_rates, C_w):
+            P = {v: scipy.linalg.expm(Q * eff[v] * r) for v in range(n_nodes) if v != root}
+            acc = 0.0
+            for assign in itertools.product(range(S), repeat=len(internal)):
+                st = dict(zip(internal, assign))
+                p = pi[st[root]]
+                for v in internal:
+                    for ch in (tree_left[v], tree_right[v]):
+                        if ch in st:
+                            p *= P[ch][st[v], st[ch]]
+                        else:
+                            mk = masks_by_node[ch, s]
+                            p *= sum(P[ch][st[v], j] for j in range(S) if (mk >> j) & 1)
+                acc += p
+            tot += w * acc
+        out[s] = np.log(tot)
+    return out
+
+
+@pytest.mark.parametrize("n", [3, 4, 5])
+def test_pruning_equals_brute_force(oracle, n):
+    rng = np.random.default_rng(40 + n)
+    t = synth.yule_tree(n, rng, 0.3)
+    Q, pi = rand_gtr(rng)
+    m = 6
+    masks = rng.integers(1, 16, size=(n, m)).astype(np.uint32)
+    mb = np.zeros((t.n_nodes, m), np.uint32)
+    mb[:n] = masks
+    cr, cw = oracle.gamma_rates(0.7, 2), np.array([0.5, 0.5])
+    p = oracle.Problem(t.left, t.right, t.root, t.blen, mb, 4, Q, pi, pi=pi, cat_rates=cr, cat_w=cw)
+    _, site = p.loglik()
+    want = _brute_force(t.left, t.right, t.root, t.blen, mb, Q, pi, 4, cr, cw, t.n_nodes)
+    np.testing.assert_allclose(site, want, rtol=1e-12)
+
+
+def test_two_taxon_closed_form(oracle):
+    """Pruning on (x:t1, y:t2) collapses to sum_i pi_i P_{i,x}(t1) P_{i,y}(t2)
+    = pi_x P_{x,y}(t1 + t2) for a reversible model."""
+    rng = np.random.default_rng(3)
+    Q, pi = rand_gtr(rng)
+    left = np.array([-1, -1, 0], np.int32)
+    right = np.array([-1, -1, 1], np.int32)
+    eff = np.array([0.13, 0.41, 0.0])
+    mb = np.array([[1, 2, 4, 8], [1, 1, 8, 2], [0, 0, 0, 0]], np.uint32)
+    p = oracle.Problem(left, right, 2, eff, mb, 4, Q, pi, pi=pi)
+    _, site = p.loglik()
+    P = scipy.linalg.expm(Q * 0.54)
+    for s, (x, y) in enumerate([(0, 0), (1, 0), (2, 3), (3, 1)]):
+        assert abs(site[s] - np.log(pi[x] * P[x, y])) < 1e-13
+
+
+def test_pulley_principle_and_missing_site(oracle):
+    """Reversible model: moving the root along the root edge leaves lnL
+    unchanged; a site missing in every taxon has lnL 0."""
+    rng = np.random.default_rng(9)
+    t = synth.yule_tree(12, rng, 0.1)
+    Q, pi = rand_gtr(rng)
+    m = 40
+    masks = (np.uint32(1) << rng.integers(0, 4, size=(12, m)).astype(np.uint32)).astype(np.uint32)
+    masks[:, 7] = 15
+    mb = np.zeros((t.n_nodes, m), np.uint32)
+    mb[:12] = masks
+    a, b = t.left[t.root], t.right[t.root]
+    tot = []
+    for frac in (0.5, 0.1, 0.9):
+        eff = t.blen.copy()
+        L = t.blen[a] + t.blen[b]
+        eff[a], eff[b] = frac * L, (1 - frac) * L
+        tt, site = oracle.Problem(t.left, t.right, t.root, eff, mb, 4, Q, pi, pi=pi).loglik()
+        tot.append(tt)
+        assert abs(site[7]) < 1e-12
+    assert max(tot) - min(tot) < 1e-9 * abs(tot[0])
+
+
+def test_site_permutation_and_gamma_limit(oracle):
+    rng = np.random.default_rng(10)
+    wl = synth.gtr_gamma_workload(n_tips=10, m=50, C=4, alpha=0.5, gap_frac=0.1, seed=10)
+    t = wl.tree
+    mb = np.zeros((t.n_nodes, 50), np.uint32)
+    mb[:10] = wl.masks
+    perm = rng.permutation(50)
+    p1 = oracle.Problem(t.left, t.right, t.root, t.blen, mb, 4, wl.Q, wl.pi, pi=wl.pi, cat_rates=wl.cat_rates)
+    p2 = oracle.Problem(t.left, t.right, t.root, t.blen, mb[:, perm], 4, wl.Q, wl.pi, pi=wl.pi,
+                        cat_rates=wl.cat_rates)
+    np.testing.assert_allclose(p1.loglik()[1][perm], p2.loglik()[1], rtol=1e-14)
+    # alpha -> infinity: gamma categories collapse to rate 1
+    big = oracle.gamma_rates(1e6, 4)
+    np.testing.assert_allclose(big, 1.0, atol=3e-3)
+    p3 = oracle.Problem(t.left, t.right, t.root, t.blen, mb, 4, wl.Q, wl.pi, pi=wl.pi, cat_rates=big)
+    p4 = oracle.Problem(t.left, t.right, t.root, t.blen, mb, 4, wl.Q, wl.pi, pi=wl.pi, cat_rates=[1.0])
+    assert abs(p3.loglik()[0] - p4.loglik()[0]) < 1e-2
+
+
+def test_nonreversible_root_is_unweighted(oracle):
+    """Non-reversible models sum the root partials without frequencies
+    (simpleML.h:133): lnL = textbook(pi uniform) + m ln S."""
+    rng = np.random.default_rng(12)
+    t = synth.yule_tree(6, rng, 0.2)
+    Q = oracle.build_Q(4, 0, [0] * 12, [1 / 3])
+    m = 10
+    mb = np.zeros((t.n_nodes, m), np.uint32)
+    mb[:6] = (np.uint32(1) << rng.integers(0, 4, size=(6, m)).astype(np.uint32))
+    t_ones, _ = oracle.Problem(t.left, t.right, t.root, t.blen, mb, 4, Q, np.ones(4), builder=oracle.PADE).loglik()
+    t_pi, _ = oracle.Problem(t.left, t.right, t.root, t.blen, mb, 4, Q, np.full(4, 0.25), builder=oracle.PADE).loglik()
+    assert abs(t_ones - (t_pi + m * np.log(4))) < 1e-10
+
+
+def test_planted_mistakes_fail(oracle):
+    """A dropped frequency factor or a transposed P must change the result:
+    the golden/brute-force checks above would catch either."""
+    rng = np.random.default_rng(21)
+    Q, pi = rand_gtr(rng)
+    P = oracle.pmatrix(Q, 0.3, oracle.EIG, pi)
+    assert np.abs(P - P.T).max() > 1e-3            # transposition is detectable
+    Q_nopi = synth.gtr_Q(np.ones(6), np.full(4, 0.25))
+    assert np.abs(oracle.pmatrix(Q_nopi, 0.3, oracle.EIG, np.full(4, .25)) - P).max() > 1e-3
+
+
+def test_simulator_distribution(oracle):
+    """orc_simulate (parity unpinned: the reference draws from time-seeded
+    rand()): tip state frequencies of a long simulation match pi."""
+    rng = np.random.default_rng(22)
+    t = synth.yule_tree(4, rng, 0.5)
+    Q, pi = rand_gtr(rng)
+    st = oracle.simulate(t.left, t.right, t.root, t.blen, 20000, Q, pi, pi, seed=5)
+    freq = np.bincount(st[0], minlength=4) / 20000
+    assert np.abs(freq - pi).max() < 0.02
