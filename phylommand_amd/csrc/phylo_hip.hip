@@ -244,7 +244,180 @@ __global__ __launch_bounds__(64 * C) void prune4_kernel(
     block_sum_store(live ? w * lnl : 0.0, partial + (size_t)prog * gridDim.x + blockIdx.x);
 }
 
-// Compensated fixed-order sum of the per-workgroup partials of each program.
+// One child of the wide-state walker: acc[i] (=|*=) sum_j P[i][j] x_j.
+// Stored/carried partials use the row-major P (wave-uniform, scalar loads);
+// tips gather columns of the transposed copy PT (a single state: one column;
+// all states: the row-sum column S; an ambiguity set: the sum of its columns).
+template <typename T, int C, int SP>
+__device__ inline void wide_child(int kind, int idx, int node, int cat, int s, int m_pad, int S, uint32_t full,
+                                  const uint32_t* __restrict__ tips, const T* plane, size_t slot_stride,
+                                  const int* scl, const double* __restrict__ P, const double* __restrict__ PT,
+                                  const T (&cA)[SP], int sA, const T (&cB)[SP], int sB, double (&acc)[SP],
+                                  int& sc, bool mul) {
+    if (kind == SRC_TIP) {
+        const uint32_t mk = tips[(size_t)idx * m_pad + s];
+        const double* cols = PT + ((size_t)node * C + cat) * (size_t)(S + 1) * SP;
+        double out[SP];
+        if (mk == full || __popc(mk) == 1) {
+            const int j = mk == full ? S : __ffs(mk) - 1;
+            const double2* col = reinterpret_cast<const double2*>(cols + (size_t)j * SP);
+#pragma unroll
+            for (int i = 0; i < SP / 2; ++i) {
+                const double2 v = col[i];
+                out[2 * i] = v.x;
+                out[2 * i + 1] = v.y;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < SP; ++i) out[i] = 0.0;
+            uint32_t rest = mk;
+            while (rest) {
+                const int j = __ffs(rest) - 1;
+                rest &= rest - 1;
+                const double2* col = reinterpret_cast<const double2*>(cols + (size_t)j * SP);
+#pragma unroll
+                for (int i = 0; i < SP / 2; ++i) {
+                    const double2 v = col[i];
+                    out[2 * i] += v.x;
+                    out[2 * i + 1] += v.y;
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < SP; ++i) acc[i] = mul ? acc[i] * out[i] : out[i];
+        return;
+    }
+    double x[SP];
+    if (kind == SRC_CLV) {
+        const T* src = plane + (size_t)idx * slot_stride;
+#pragma unroll
+        for (int q = 0; q < SP / 4; ++q) load4(src + 4 * q, x + 4 * q);
+        sc += scl[(size_t)idx * m_pad + s];
+    } else if (kind == SRC_CARRY_A) {
+#pragma unroll
+        for (int j = 0; j < SP; ++j) x[j] = (double)cA[j];
+        sc += sA;
+    } else {
+#pragma unroll
+        for (int j = 0; j < SP; ++j) x[j] = (double)cB[j];
+        sc += sB;
+    }
+    const double* Pn = P + ((size_t)node * C + cat) * SP * SP;
+#pragma unroll
+    for (int i = 0; i < SP; ++i) {
+        double a = 0.0;
+#pragma unroll
+        for (int j = 0; j < SP; ++j) a = fma(Pn[i * SP + j], x[j], a);
+        acc[i] = mul ? acc[i] * a : a;
+    }
+}
+
+// The pruning walker for 4 < S <= 32 (protein and other wide alphabets):
+// the DNA walker's structure with S padded to SP (a multiple of 4) states
+// per partial, 32-bit tip masks, and FP64 arithmetic on unpadded-in-effect
+// rows (padded rows and columns of P are zero).
+template <typename T, int C, int SP>
+__global__ __launch_bounds__(64 * C) void prune_wide_kernel(
+    const Op* __restrict__ ops, const int* __restrict__ prog_begin, const uint32_t* __restrict__ tips,
+    T* clv, int* scl, const double* __restrict__ P, const double* __restrict__ PT, const double* __restrict__ roots,
+    const double* __restrict__ site_w, double* __restrict__ site_lnl, double* __restrict__ partial,
+    int m, int m_pad, int S, uint32_t full) {
+    __shared__ int xexp[2][C][64];
+    __shared__ double xroot[C][64];
+    const int lane = threadIdx.x & 63;
+    const int cat = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int s = blockIdx.x * 64 + lane;
+    const int prog = blockIdx.y;
+    const int k0 = prog_begin[prog], k1 = prog_begin[prog + 1];
+    T cA[SP], cB[SP];
+#pragma unroll
+    for (int i = 0; i < SP; ++i) { cA[i] = T(0); cB[i] = T(0); }
+    int sA = 0, sB = 0;
+    double lnl = 0.0;
+    T* plane = clv + ((size_t)cat * m_pad + s) * SP;
+    const size_t slot_stride = (size_t)C * m_pad * SP;
+    for (int k = k0; k < k1; ++k) {
+        const Op op = ops[k];
+        double acc[SP];
+        int sc = 0;
+        wide_child<T, C, SP>(op.l_kind, op.l_idx, op.l_node, cat, s, m_pad, S, full, tips, plane, slot_stride, scl,
+                             P, PT, cA, sA, cB, sB, acc, sc, false);
+        if (op.r_kind != SRC_NONE)
+            wide_child<T, C, SP>(op.r_kind, op.r_idx, op.r_node, cat, s, m_pad, S, full, tips, plane, slot_stride,
+                                 scl, P, PT, cA, sA, cB, sB, acc, sc, true);
+        double mx = 0.0;
+#pragma unroll
+        for (int i = 0; i < SP; ++i) mx = fmax(mx, acc[i]);
+        int e = INT_MIN;
+        if (mx > 0.0) (void)frexp(mx, &e);
+        if (C > 1) {
+            const int buf = k & 1;
+            xexp[buf][cat][lane] = e;
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < C; ++c) e = max(e, xexp[buf][c][lane]);
+        }
+        if (e == INT_MIN) e = 0;
+        sc += e;
+        T r[SP];
+#pragma unroll
+        for (int i = 0; i < SP; ++i) {
+            acc[i] = ldexp(acc[i], -e);
+            r[i] = (T)acc[i];
+        }
+        if (op.out_slot >= 0) {
+            T* dst = plane + (size_t)op.out_slot * slot_stride;
+#pragma unroll
+            for (int q = 0; q < SP / 4; ++q) store4(dst + 4 * q, r + 4 * q);
+            if (cat == 0) scl[(size_t)op.out_slot * m_pad + s] = sc;
+        }
+        if (op.flags & OP_TO_B) {
+#pragma unroll
+            for (int i = 0; i < SP; ++i) cB[i] = r[i];
+            sB = sc;
+        } else {
+#pragma unroll
+            for (int i = 0; i < SP; ++i) cA[i] = r[i];
+            sA = sc;
+        }
+        if (op.flags & OP_ROOT) {
+            double lc = 0.0;
+#pragma unroll
+            for (int i = 0; i < SP; ++i) lc = fma(roots[i], acc[i], lc);
+            lc *= roots[kRootCatOff + cat];
+            if (C > 1) {
+                xroot[cat][lane] = lc;
+                __syncthreads();
+                lc = 0.0;
+#pragma unroll
+                for (int c = 0; c < C; ++c) lc += xroot[c][lane];
+                __syncthreads();
+            }
+            lnl = log(lc) + sc * kLn2;
+        }
+    }
+    const bool live = s < m && cat == 0;
+    if (live && site_lnl) site_lnl[(size_t)prog * m + s] = lnl;
+    const double w = live ? (site_w ? site_w[s] : 1.0) : 0.0;
+    block_sum_store(live ? w * lnl : 0.0, partial + (size_t)prog * gridDim.x + blockIdx.x);
+}
+
+// PT[node][cat][j][i] = P[node][cat][i][j] for j < S, PT[..][S][i] = sum_j P[i][j].
+__global__ void ptranspose_kernel(const double* __restrict__ P, int S, int SP, long n_mat, double* __restrict__ PT) {
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long per = (long)(S + 1) * SP;
+    if (gid >= n_mat * per) return;
+    const long mat = gid / per;
+    const int j = (gid % per) / SP;
+    const int i = gid % SP;
+    const double* Pm = P + mat * SP * SP;
+    double v = 0.0;
+    if (j < S) v = Pm[i * SP + j];
+    else
+        for (int q = 0; q < S; ++q) v += Pm[i * SP + q];
+    PT[gid] = v;
+}
+
 // Compensated fixed-order sum of the per-workgroup partials of each program
 // (one 256-thread block per program: strided Kahan sums, then a fixed tree).
 __global__ __launch_bounds__(256) void final_sum_kernel(const double* __restrict__ partial, int n_part,
@@ -334,7 +507,8 @@ struct phylo_ctx {
     int n_slots = 0;
 
     // device buffers
-    uint8_t* d_tips = nullptr; size_t tips_bytes = 0;
+    void* d_tips = nullptr; size_t tips_bytes = 0;     // uint8 codes (S <= 4) or uint32 masks
+    double* d_PT = nullptr; size_t PT_bytes = 0;       // transposed P + row sums (wide states)
     void* d_clv = nullptr; size_t clv_bytes = 0;
     int* d_scl = nullptr; size_t scl_bytes = 0;
     double* d_P = nullptr; size_t P_bytes = 0;
@@ -374,7 +548,16 @@ hipError_t ensure(T*& ptr, size_t& have, size_t need) {
     return e;
 }
 
-int padded_states(int S) { return S <= 4 ? 4 : 32; }
+// States per stored partial: 4 for the DNA walker, else the next supported
+// multiple of 4 (padded rows/columns of P are zero).
+int padded_states(int S) {
+    if (S <= 4) return 4;
+    if (S <= 8) return 8;
+    if (S <= 16) return 16;
+    if (S <= 20) return 20;
+    if (S <= 24) return 24;
+    return 32;
+}
 
 // Builds the full-evaluation program: internal nodes in the reference's
 // postorder, carry A holding the previous op's result.
@@ -470,6 +653,14 @@ int upload_pmatrices(phylo_ctx* c) {
         HIPCHK(c, hipMemcpyAsync(c->d_P, P.data(), P.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
+    if (Sp > 4) {
+        const long n_mat = (long)c->n_nodes * c->C;
+        const long nPT = n_mat * (c->S_model + 1) * Sp;
+        HIPCHK(c, ensure(c->d_PT, c->PT_bytes, (size_t)nPT * sizeof(double)));
+        hipLaunchKernelGGL(ptranspose_kernel, dim3((unsigned)((nPT + 255) / 256)), dim3(256), 0, c->stream, c->d_P,
+                           c->S_model, Sp, n_mat, c->d_PT);
+        HIPCHK(c, hipGetLastError());
+    }
     c->dirty_P = false;
     return PHYLO_OK;
 }
@@ -480,17 +671,27 @@ int prepare(phylo_ctx* c) {
     if (!c->have_eff) return fail(c, PHYLO_ERR_STATE, "phylo_set_edge_lengths has not been called");
     if (!c->have_model) return fail(c, PHYLO_ERR_STATE, "phylo_set_model has not been called");
     if (c->S_model != c->S) return fail(c, PHYLO_ERR_ARG, "model and tip data disagree on the number of states");
-    if (c->S > 4) return fail(c, PHYLO_ERR_UNSUPPORTED, "more than 4 states needs the wide-state kernel");
     HIPCHK(c, hipSetDevice(c->device));
     if (c->dirty_tips) {
-        // [taxon][m_pad] 4-bit codes; padding sites are all-states (finite, ignored in the sum)
-        std::vector<uint8_t> codes((size_t)c->n_taxa * c->m_pad, 0x0F);
-        for (int t = 0; t < c->n_taxa; ++t)
-            for (int s = 0; s < c->m; ++s)
-                codes[(size_t)t * c->m_pad + s] = (uint8_t)(c->masks[(size_t)t * c->m + s] & 0x0Fu);
-        HIPCHK(c, ensure(c->d_tips, c->tips_bytes, codes.size()));
-        HIPCHK(c, hipMemcpyAsync(c->d_tips, codes.data(), codes.size(), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        // [taxon][m_pad]; padding sites are all-states (finite, ignored in the sum)
+        if (c->S <= 4) {
+            std::vector<uint8_t> codes((size_t)c->n_taxa * c->m_pad, 0x0F);
+            for (int t = 0; t < c->n_taxa; ++t)
+                for (int s = 0; s < c->m; ++s)
+                    codes[(size_t)t * c->m_pad + s] = (uint8_t)(c->masks[(size_t)t * c->m + s] & 0x0Fu);
+            HIPCHK(c, ensure(c->d_tips, c->tips_bytes, codes.size()));
+            HIPCHK(c, hipMemcpyAsync(c->d_tips, codes.data(), codes.size(), hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        } else {
+            const uint32_t full = c->S == 32 ? 0xFFFFFFFFu : ((1u << c->S) - 1u);
+            std::vector<uint32_t> codes((size_t)c->n_taxa * c->m_pad, full);
+            for (int t = 0; t < c->n_taxa; ++t)
+                for (int s = 0; s < c->m; ++s) codes[(size_t)t * c->m_pad + s] = c->masks[(size_t)t * c->m + s];
+            HIPCHK(c, ensure(c->d_tips, c->tips_bytes, codes.size() * sizeof(uint32_t)));
+            HIPCHK(c, hipMemcpyAsync(c->d_tips, codes.data(), codes.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                     c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
         c->dirty_tips = false;
     }
     if (c->dirty_site_w) {
@@ -512,7 +713,8 @@ int prepare(phylo_ctx* c) {
         c->dirty_topo = false;
     }
     const size_t elt = c->prec == PHYLO_PREC_FP64 ? sizeof(double) : sizeof(float);
-    HIPCHK(c, ensure(c->d_clv, c->clv_bytes, (size_t)std::max(c->n_slots, 1) * c->C * c->m_pad * 4 * elt));
+    HIPCHK(c, ensure(c->d_clv, c->clv_bytes,
+                     (size_t)std::max(c->n_slots, 1) * c->C * c->m_pad * padded_states(c->S) * elt));
     HIPCHK(c, ensure(c->d_scl, c->scl_bytes, (size_t)std::max(c->n_slots, 1) * c->m_pad * sizeof(int)));
     if (c->dirty_P) {
         int rc = upload_pmatrices(c);
@@ -530,9 +732,42 @@ int prepare(phylo_ctx* c) {
     return PHYLO_OK;
 }
 
+template <typename T, int C, int SP>
+void launch_wide(phylo_ctx* c, dim3 grid, double* site_lnl) {
+    const uint32_t full = c->S == 32 ? 0xFFFFFFFFu : ((1u << c->S) - 1u);
+    hipLaunchKernelGGL((prune_wide_kernel<T, C, SP>), grid, dim3(64 * C), 0, c->stream, c->d_ops, c->d_prog,
+                       reinterpret_cast<const uint32_t*>(c->d_tips), reinterpret_cast<T*>(c->d_clv), c->d_scl, c->d_P,
+                       c->d_PT, c->d_roots, c->have_site_w ? c->d_sitew : nullptr, site_lnl, c->d_partial, c->m,
+                       c->m_pad, c->S, full);
+}
+
+template <typename T, int C>
+int dispatch_wide_sp(phylo_ctx* c, dim3 grid, double* site_lnl) {
+    switch (padded_states(c->S)) {
+    case 8: launch_wide<T, C, 8>(c, grid, site_lnl); break;
+    case 16: launch_wide<T, C, 16>(c, grid, site_lnl); break;
+    case 20: launch_wide<T, C, 20>(c, grid, site_lnl); break;
+    case 24: launch_wide<T, C, 24>(c, grid, site_lnl); break;
+    default: launch_wide<T, C, 32>(c, grid, site_lnl); break;
+    }
+    return PHYLO_OK;
+}
+
+template <typename T>
+int dispatch_wide(phylo_ctx* c, dim3 grid, double* site_lnl) {
+    switch (c->C) {
+    case 1: return dispatch_wide_sp<T, 1>(c, grid, site_lnl);
+    case 2: return dispatch_wide_sp<T, 2>(c, grid, site_lnl);
+    case 4: return dispatch_wide_sp<T, 4>(c, grid, site_lnl);
+    case 8: return dispatch_wide_sp<T, 8>(c, grid, site_lnl);
+    default: return fail(c, PHYLO_ERR_UNSUPPORTED, "number of rate categories must be 1, 2, 4 or 8");
+    }
+}
+
 template <typename T, int C>
 void launch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
-    hipLaunchKernelGGL((prune4_kernel<T, C>), grid, dim3(64 * C), 0, c->stream, c->d_ops, c->d_prog, c->d_tips,
+    hipLaunchKernelGGL((prune4_kernel<T, C>), grid, dim3(64 * C), 0, c->stream, c->d_ops, c->d_prog,
+                       reinterpret_cast<const uint8_t*>(c->d_tips),
                        reinterpret_cast<T*>(c->d_clv), c->d_scl, c->d_P, c->d_roots,
                        c->have_site_w ? c->d_sitew : nullptr, site_lnl, c->d_partial, c->m, c->m_pad);
 }
@@ -565,7 +800,11 @@ int run_programs(phylo_ctx* c, int n_prog, double* totals, double* site_lnl_host
     dim3 grid(nblk, n_prog);
     double* dsite = site_lnl_host ? c->d_sitelnl : nullptr;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    int rc = c->prec == PHYLO_PREC_FP64 ? dispatch_prune4<double>(c, grid, dsite) : dispatch_prune4<float>(c, grid, dsite);
+    int rc;
+    if (c->S <= 4)
+        rc = c->prec == PHYLO_PREC_FP64 ? dispatch_prune4<double>(c, grid, dsite) : dispatch_prune4<float>(c, grid, dsite);
+    else
+        rc = c->prec == PHYLO_PREC_FP64 ? dispatch_wide<double>(c, grid, dsite) : dispatch_wide<float>(c, grid, dsite);
     if (rc) return rc;
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -613,7 +852,7 @@ void phylo_destroy(phylo_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void* bufs[] = {c->d_tips, c->d_clv, c->d_scl, c->d_P, c->d_ops, c->d_prog, c->d_roots, c->d_sitew,
+    void* bufs[] = {c->d_tips, c->d_PT, c->d_clv, c->d_scl, c->d_P, c->d_ops, c->d_prog, c->d_roots, c->d_sitew,
                     c->d_sitelnl, c->d_partial, c->d_total, c->d_eig, c->d_eff};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -771,7 +1010,7 @@ int phylo_node_partials(phylo_ctx* c, int node, double* clv, int32_t* scale) {
     if (!c->d_clv || c->ops.empty()) return fail(c, PHYLO_ERR_STATE, "node_partials before loglik");
     if (node < c->n_taxa || node >= c->n_nodes) return fail(c, PHYLO_ERR_ARG, "node_partials: not an internal node");
     const int slot = node - c->n_taxa;
-    const int Sp = 4;
+    const int Sp = padded_states(c->S);
     const size_t n = (size_t)c->C * c->m_pad * Sp;
     std::vector<int> sc(c->m_pad);
     HIPCHK(c, hipSetDevice(c->device));

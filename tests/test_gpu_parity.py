@@ -37,3 +37,31 @@ def test_dna_loglik_matches_oracle(oracle, n, m, C, alpha, gap, seed, prec):
         assert abs(tot - ref_tot) <= 1e-3, abs(tot - ref_tot)
     # the total is the compensated sum of the per-site values
     assert abs(tot - np.sum(site)) <= 1e-9 * abs(tot)
+
+
+WIDE_CASES = [
+    # n_tips, m, S, C, alpha, gap, seed
+    (8, 500, 20, 1, 1.0, 0.0, 11),
+    (64, 20000, 20, 4, 1.0, 0.03, 12),     # the 20-state protein configuration
+    (16, 1500, 6, 2, 0.8, 0.05, 13),
+    (12, 900, 23, 4, 0.5, 0.05, 14),
+    (10, 700, 32, 8, 1.2, 0.02, 15),
+    (20, 1200, 13, 4, 0.6, 0.05, 16),
+]
+
+
+@pytest.mark.parametrize("n,m,S,C,alpha,gap,seed", WIDE_CASES)
+@pytest.mark.parametrize("prec", [0, 1])
+def test_wide_state_loglik_matches_oracle(oracle, n, m, S, C, alpha, gap, seed, prec):
+    print(f"wide case n={n} m={m} S={S} C={C} prec={prec}", flush=True)
+    wl = synth.gtr_gamma_workload(n_tips=n, m=m, S=S, C=C, alpha=alpha, gap_frac=gap, seed=seed,
+                                  rate_dist="gamma", freq_conc=2.0)
+    ref_tot, ref_site = oracle_problem(oracle, wl).loglik()
+    with gpu_context(wl, precision=prec) as ctx:
+        tot, site = ctx.loglik(per_site=True)
+    rel = np.abs(site - ref_site) / np.abs(ref_site)
+    if prec == 1:
+        assert rel.max() <= 1e-9, rel.max()
+    else:
+        assert rel.max() <= 1e-5, rel.max()
+        assert abs(tot - ref_tot) <= 1e-3, abs(tot - ref_tot)
