@@ -111,6 +111,25 @@ int phylo_node_partials(phylo_ctx *ctx, int node, double *clv, int32_t *scale);
  * Replaces: sub_model::get_P_value after set_P_matrix (sub_model.h:46-50). */
 int phylo_get_pmatrix(phylo_ctx *ctx, int node, double *P);
 
+/* Scores the NNI neighbourhood of the current topology.  mode 0: the trees
+ * treebender --nni all emits after the original (internal branches 2..n-3,
+ * left then right swap, 2*(n-4) variants; treebender.cpp:897-915 with
+ * tree::nni / tree::interchange, tree.cpp:2480-2548); mode 1: the full
+ * unrooted neighbourhood, 2*(n-3) variants.  Each variant recomputes only the
+ * two re-attached nodes and the path to the root against the stored partials
+ * of the current tree (evaluated first if stale), all variants in one launch.
+ * Branch lengths move with their subtrees, as in interchange.  lnl: one total
+ * per variant (capacity max_variants); node/side (nullable): the interchanged
+ * node and side (0 left, 1 right); *n_variants: the count; base_lnl
+ * (nullable): the unchanged tree's total.  Replaces: scoring each
+ * treebender --nni tree from scratch through treeator (treeator.cpp:705-990). */
+int phylo_nni_score(phylo_ctx *ctx, int mode, int max_variants, double *lnl, int32_t *node, int32_t *side,
+                    int *n_variants, double *base_lnl);
+
+/* Number of partial-recompute ops the last phylo_nni_score ran (updates
+ * per site and category), for throughput accounting. */
+int phylo_nni_ops(phylo_ctx *ctx, long *n_ops);
+
 /* Waits for all work on the context's device (hipDeviceSynchronize). */
 int phylo_synchronize(phylo_ctx *ctx);
 

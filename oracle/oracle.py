@@ -68,6 +68,8 @@ def lib():
         L.orc_node_clvs.argtypes = common + [_dp, _dp, _ip, _dp, _dp]
         L.orc_node_partials.argtypes = common + [_dp, ctypes.c_int, _dp]
         L.orc_nni_reference.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+        L.orc_nni_list.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, ctypes.c_int, _ip, _ip]
+        L.orc_nni_apply.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, ctypes.c_int, ctypes.c_int, _ip, _ip]
         L.orc_simulate.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp,
                                    _dp, ctypes.c_int, _dp, ctypes.c_uint64, _ip]
         L.orc_time_reference_cost.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, _dp, ctypes.c_int, ctypes.c_int,
@@ -283,3 +285,23 @@ def simulate(left, right, root, blen, m, Q, pi, freqs_root, cat_rates=(1.0,), se
                        _d(freqs_root).ctypes.data_as(_dp), len(cr), cr.ctypes.data_as(_dp), seed,
                        states.ctypes.data_as(_ip))
     return states
+
+
+def nni_list(left, right, root, mode=0):
+    """[(node, side)] of the NNI variants (mode 0: treebender --nni all; 1: full
+    unrooted neighbourhood)."""
+    n = len(left)
+    nodes = np.zeros(2 * n, np.int32)
+    sides = np.zeros(2 * n, np.int32)
+    k = lib().orc_nni_list(n, int(root), _i(left).ctypes.data_as(_ip), _i(right).ctypes.data_as(_ip), mode,
+                           nodes.ctypes.data_as(_ip), sides.ctypes.data_as(_ip))
+    return list(zip(nodes[:k].tolist(), sides[:k].tolist()))
+
+
+def nni_apply(left, right, root, node, side):
+    n = len(left)
+    ol = np.zeros(n, np.int32)
+    orr = np.zeros(n, np.int32)
+    lib().orc_nni_apply(n, int(root), _i(left).ctypes.data_as(_ip), _i(right).ctypes.data_as(_ip), int(node),
+                        int(side), ol.ctypes.data_as(_ip), orr.ctypes.data_as(_ip))
+    return ol, orr

@@ -959,6 +959,58 @@ int orc_nni_reference(const char *newick, char *out, int cap) {
     return count;
 }
 
+static orc_tree *tree_from_arrays(int n_nodes, int root, const int *left, const int *right) {
+    orc_tree *t = calloc(1, sizeof(orc_tree));
+    for (int i = 0; i < n_nodes; ++i) tree_new_node(t);
+    t->root = root;
+    for (int i = 0; i < n_nodes; ++i) { t->left[i] = left[i]; t->right[i] = right[i]; }
+    for (int i = 0; i < n_nodes; ++i) {
+        if (left[i] >= 0) t->parent[left[i]] = i;
+        if (right[i] >= 0) t->parent[right[i]] = i;
+    }
+    return t;
+}
+
+/* NNI variants of an array tree as (node, side) pairs, side 0 = left swap.
+ * mode 0: treebender --nni all (branches 2..n-3 through
+ * get_internal_branch_by_number_unrooted, treebender.cpp:897-915);
+ * mode 1: every internal edge of the unrooted tree (both root children
+ * count as one edge), 2*(n-3) variants.  Returns the count. */
+int orc_nni_list(int n_nodes, int root, const int *left, const int *right, int mode, int *out_node, int *out_side) {
+    orc_tree *t = tree_from_arrays(n_nodes, root, left, right);
+    int ntips = n_sub_tips(t, root), k = 0;
+    if (mode == 0) {
+        if (ntips >= 4)
+            for (int b = 2; b <= ntips - 3; ++b) {
+                int v = branch_by_number(t, b);
+                for (int side = 0; side < 2; ++side) { out_node[k] = v; out_side[k] = side; ++k; }
+            }
+    } else {
+        int a = left[root], bb = right[root];
+        int a_int = a >= 0 && left[a] >= 0, b_int = bb >= 0 && left[bb] >= 0;
+        for (int v = 0; v < n_nodes; ++v) {
+            if (v == root || left[v] < 0) continue;
+            if (t->parent[v] == root) {
+                if (!(a_int && b_int)) continue;      /* root edge is not internal */
+                if (v == bb) continue;                /* count the root edge once */
+            }
+            for (int side = 0; side < 2; ++side) { out_node[k] = v; out_side[k] = side; ++k; }
+        }
+    }
+    orc_tree_free(t);
+    return k;
+}
+
+/* Applies tree::interchange(node, side == 0) (tree.cpp:2480-2518) to copies
+ * of the arrays. */
+void orc_nni_apply(int n_nodes, int root, const int *left, const int *right, int node, int side,
+                   int *out_left, int *out_right) {
+    orc_tree *t = tree_from_arrays(n_nodes, root, left, right);
+    if (node >= 0) interchange(t, node, side == 0);
+    for (int i = 0; i < n_nodes; ++i) { out_left[i] = t->left[i]; out_right[i] = t->right[i]; }
+    orc_tree_free(t);
+}
+
 /* ------------------------------------------------------------------ */
 /* Forward simulation (simpleML.cpp:233-292), seeded                    */
 /* ------------------------------------------------------------------ */

@@ -42,6 +42,9 @@ SIGNATURES = [
     ("phylo_loglik", _c_int, [_c_void_p, _c_dbl_p, _c_dbl_p]),
     ("phylo_node_partials", _c_int, [_c_void_p, _c_int, _c_dbl_p, _c_int_p]),
     ("phylo_get_pmatrix", _c_int, [_c_void_p, _c_int, _c_dbl_p]),
+    ("phylo_nni_score", _c_int, [_c_void_p, _c_int, _c_int, _c_dbl_p, _c_int_p, _c_int_p,
+                                 ctypes.POINTER(ctypes.c_int), _c_dbl_p]),
+    ("phylo_nni_ops", _c_int, [_c_void_p, ctypes.POINTER(ctypes.c_long)]),
     ("phylo_synchronize", _c_int, [_c_void_p]),
     ("phylo_last_kernel_ms", _c_int, [_c_void_p, _c_dbl_p]),
 ]
@@ -174,6 +177,24 @@ class Context:
         P = np.zeros((n_cat, n_states, n_states))
         self._check(self.lib.phylo_get_pmatrix(self.h, int(node), _dp(P)))
         return P
+
+    def nni_score(self, mode: int = 1):
+        """Returns (base_lnl, [(node, side, lnl)]) for every NNI variant."""
+        nv = ctypes.c_int(0)
+        base = np.zeros(1)
+        self._check(self.lib.phylo_nni_score(self.h, mode, 0, None, None, None, ctypes.byref(nv), _dp(base)))
+        n = nv.value
+        lnl = np.zeros(max(n, 1))
+        nodes = np.zeros(max(n, 1), np.int32)
+        sides = np.zeros(max(n, 1), np.int32)
+        self._check(self.lib.phylo_nni_score(self.h, mode, n, _dp(lnl), _ip(nodes), _ip(sides), ctypes.byref(nv),
+                                             _dp(base)))
+        return float(base[0]), list(zip(nodes[:n].tolist(), sides[:n].tolist(), lnl[:n].tolist()))
+
+    def nni_ops(self) -> int:
+        n = ctypes.c_long(0)
+        self._check(self.lib.phylo_nni_ops(self.h, ctypes.byref(n)))
+        return n.value
 
     def synchronize(self):
         self._check(self.lib.phylo_synchronize(self.h))

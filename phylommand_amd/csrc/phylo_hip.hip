@@ -500,6 +500,8 @@ struct phylo_ctx {
     std::vector<double> cat_rate{1.0}, cat_w{1.0};
     bool dirty_P = true;
     bool dirty_roots = true;
+    double base_total = 0.0;
+    long nni_ops_last = 0;
 
     // compiled program
     std::vector<Op> ops;
@@ -514,6 +516,11 @@ struct phylo_ctx {
     double* d_P = nullptr; size_t P_bytes = 0;
     Op* d_ops = nullptr; size_t ops_bytes = 0;
     int* d_prog = nullptr; size_t prog_bytes = 0;
+    Op* d_nni_ops = nullptr; size_t nni_ops_bytes = 0;   // NNI partial-recompute programs
+    int* d_nni_prog = nullptr; size_t nni_prog_bytes = 0;
+    const Op* run_ops = nullptr;                          // program set of the launch in flight
+    const int* run_prog = nullptr;
+    bool clv_valid = false;                               // stored partials match the current inputs
     double* d_roots = nullptr;
     double* d_sitew = nullptr; size_t sitew_bytes = 0;
     double* d_sitelnl = nullptr; size_t sitelnl_bytes = 0;
@@ -735,7 +742,7 @@ int prepare(phylo_ctx* c) {
 template <typename T, int C, int SP>
 void launch_wide(phylo_ctx* c, dim3 grid, double* site_lnl) {
     const uint32_t full = c->S == 32 ? 0xFFFFFFFFu : ((1u << c->S) - 1u);
-    hipLaunchKernelGGL((prune_wide_kernel<T, C, SP>), grid, dim3(64 * C), 0, c->stream, c->d_ops, c->d_prog,
+    hipLaunchKernelGGL((prune_wide_kernel<T, C, SP>), grid, dim3(64 * C), 0, c->stream, c->run_ops, c->run_prog,
                        reinterpret_cast<const uint32_t*>(c->d_tips), reinterpret_cast<T*>(c->d_clv), c->d_scl, c->d_P,
                        c->d_PT, c->d_roots, c->have_site_w ? c->d_sitew : nullptr, site_lnl, c->d_partial, c->m,
                        c->m_pad, c->S, full);
@@ -766,7 +773,7 @@ int dispatch_wide(phylo_ctx* c, dim3 grid, double* site_lnl) {
 
 template <typename T, int C>
 void launch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
-    hipLaunchKernelGGL((prune4_kernel<T, C>), grid, dim3(64 * C), 0, c->stream, c->d_ops, c->d_prog,
+    hipLaunchKernelGGL((prune4_kernel<T, C>), grid, dim3(64 * C), 0, c->stream, c->run_ops, c->run_prog,
                        reinterpret_cast<const uint8_t*>(c->d_tips),
                        reinterpret_cast<T*>(c->d_clv), c->d_scl, c->d_P, c->d_roots,
                        c->have_site_w ? c->d_sitew : nullptr, site_lnl, c->d_partial, c->m, c->m_pad);
@@ -786,7 +793,10 @@ int dispatch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
 
 // Launches the program set currently in c->ops / c->prog_begin and returns
 // one total per program in totals (n_prog values); site_lnl [n_prog][m] is optional.
-int run_programs(phylo_ctx* c, int n_prog, double* totals, double* site_lnl_host) {
+int run_programs(phylo_ctx* c, const Op* d_ops, const int* d_prog, int n_prog, double* totals,
+                 double* site_lnl_host) {
+    c->run_ops = d_ops;
+    c->run_prog = d_prog;
     const int nblk = c->m_pad / 64;
     HIPCHK(c, ensure(c->d_partial, c->partial_bytes, (size_t)n_prog * nblk * sizeof(double)));
     HIPCHK(c, ensure(c->d_total, c->total_bytes, (size_t)n_prog * sizeof(double)));
@@ -821,6 +831,133 @@ int run_programs(phylo_ctx* c, int n_prog, double* totals, double* site_lnl_host
     return PHYLO_OK;
 }
 
+// ---------------------------------------------------------------------------
+// NNI neighbourhoods (tree.cpp:2468-2548, treebender.cpp:897-915)
+// ---------------------------------------------------------------------------
+
+std::vector<int> parents_of(const phylo_ctx* c) {
+    std::vector<int> par(c->n_nodes, -1);
+    for (int v = 0; v < c->n_nodes; ++v) {
+        if (c->left[v] >= 0) par[c->left[v]] = v;
+        if (c->right[v] >= 0) par[c->right[v]] = v;
+    }
+    return par;
+}
+
+int count_tips(const phylo_ctx* c, int v) {
+    if (v < 0) return 0;
+    if (c->left[v] < 0 && c->right[v] < 0) return 1;
+    return count_tips(c, c->left[v]) + count_tips(c, c->right[v]);
+}
+
+// Variants as (node, side): mode 0 follows treebender --nni all (internal
+// branches 2..n-3 numbered in preorder by get_internal_branch_by_number_unrooted,
+// tree.cpp:2455-2478; left then right swap); mode 1 is every internal edge of
+// the unrooted tree, both root children counting as one edge.
+void enumerate_nni(const phylo_ctx* c, int mode, std::vector<int>& nodes, std::vector<int>& sides) {
+    nodes.clear();
+    sides.clear();
+    const int r = c->root;
+    const int a = c->left[r], b = c->right[r];
+    const auto internal = [&](int v) { return v >= 0 && c->left[v] >= 0; };
+    if (mode == 0) {
+        const int ntips = count_tips(c, r);
+        int start = r, offset = 1;
+        if (count_tips(c, a) < 2) { start = b; offset = 0; }
+        else if (count_tips(c, b) < 2) { start = a; offset = 0; }
+        std::vector<int> pre, stack;
+        if (start >= 0) stack.push_back(start);
+        while (!stack.empty()) {
+            int v = stack.back();
+            stack.pop_back();
+            if (!internal(v)) continue;
+            pre.push_back(v);
+            if (c->right[v] >= 0) stack.push_back(c->right[v]);
+            stack.push_back(c->left[v]);
+        }
+        for (int br = 2; br <= ntips - 3; ++br) {
+            const int idx = br - 1 + offset;
+            const int v = idx < (int)pre.size() ? pre[idx] : -1;
+            for (int side = 0; side < 2; ++side) { nodes.push_back(v); sides.push_back(side); }
+        }
+    } else {
+        const std::vector<int> par = parents_of(c);
+        for (int v = 0; v < c->n_nodes; ++v) {
+            if (v == r || !internal(v)) continue;
+            if (par[v] == r && (!(internal(a) && internal(b)) || v == b)) continue;
+            for (int side = 0; side < 2; ++side) { nodes.push_back(v); sides.push_back(side); }
+        }
+    }
+}
+
+// Appends the partial-recompute program of tree::interchange(v, side == 0)
+// (tree.cpp:2480-2518): the two nodes whose children change, then the path to
+// the root, each child coming from a stored partial, a tip or a register
+// carry.  Returns false when the interchange is a no-op (no swap partner).
+bool append_nni_program(const phylo_ctx* c, const std::vector<int>& par, int v, int side, std::vector<Op>& ops) {
+    if (v < 0 || v == c->root || c->left[v] < 0) return false;
+    const int p = par[v], r = c->root;
+    const auto mem = [&](int x, int& kind, int& idx) {
+        if (x < c->n_taxa) { kind = SRC_TIP; idx = x; }
+        else { kind = SRC_CLV; idx = x - c->n_taxa; }
+    };
+    const int sib = c->left[p] == v ? c->right[p] : c->left[p];
+    int partner = -1;
+    if (p != r) partner = sib;
+    else if (sib >= 0 && c->left[sib] >= 0) partner = c->left[sib];
+    if (partner < 0) return false;
+    const int tmp = side == 0 ? c->left[v] : c->right[v];
+    // v with `partner` in place of `tmp`
+    Op o1{};
+    const int vl = side == 0 ? partner : c->left[v], vr = side == 0 ? c->right[v] : partner;
+    o1.l_node = vl; o1.r_node = vr;
+    mem(vl, o1.l_kind, o1.l_idx);
+    mem(vr, o1.r_kind, o1.r_idx);
+    o1.out_slot = -1;
+    o1.flags = 0;
+    ops.push_back(o1);
+    const int pp = p != r ? p : sib;            // partner's old parent
+    // pp with `tmp` in place of `partner` (and v carried when pp == p)
+    Op o2{};
+    const int ql = c->left[pp] == partner ? tmp : c->left[pp];
+    const int qr = c->right[pp] == partner ? tmp : c->right[pp];
+    o2.l_node = ql; o2.r_node = qr;
+    if (ql == v) { o2.l_kind = SRC_CARRY_A; o2.l_idx = 0; } else mem(ql, o2.l_kind, o2.l_idx);
+    if (qr == v) { o2.r_kind = SRC_CARRY_A; o2.r_idx = 0; } else mem(qr, o2.r_kind, o2.r_idx);
+    o2.out_slot = -1;
+    if (p != r) {
+        o2.flags = 0;
+        ops.push_back(o2);
+        int cur = p;
+        while (cur != r) {
+            const int up = par[cur];
+            Op o{};
+            o.l_node = c->left[up];
+            o.r_node = c->right[up];
+            if (c->left[up] == cur) { o.l_kind = SRC_CARRY_A; o.l_idx = 0; mem(c->right[up], o.r_kind, o.r_idx); }
+            else { o.r_kind = SRC_CARRY_A; o.r_idx = 0; mem(c->left[up], o.l_kind, o.l_idx); }
+            o.out_slot = -1;
+            o.flags = up == r ? OP_ROOT : 0;
+            ops.push_back(o);
+            cur = up;
+        }
+    } else {
+        // at the root: v is carried in A, its sibling in B, then the root
+        o2.flags = OP_TO_B;
+        ops.push_back(o2);
+        Op o3{};
+        o3.l_node = c->left[r];
+        o3.r_node = c->right[r];
+        o3.l_kind = c->left[r] == v ? SRC_CARRY_A : SRC_CARRY_B;
+        o3.r_kind = c->right[r] == v ? SRC_CARRY_A : SRC_CARRY_B;
+        o3.l_idx = o3.r_idx = 0;
+        o3.out_slot = -1;
+        o3.flags = OP_ROOT;
+        ops.push_back(o3);
+    }
+    return true;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -852,7 +989,8 @@ void phylo_destroy(phylo_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void* bufs[] = {c->d_tips, c->d_PT, c->d_clv, c->d_scl, c->d_P, c->d_ops, c->d_prog, c->d_roots, c->d_sitew,
+    void* bufs[] = {c->d_tips, c->d_PT, c->d_clv, c->d_scl, c->d_P, c->d_ops, c->d_prog, c->d_nni_ops, c->d_nni_prog,
+                    c->d_roots, c->d_sitew,
                     c->d_sitelnl, c->d_partial, c->d_total, c->d_eig, c->d_eff};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -870,6 +1008,7 @@ int phylo_set_precision(phylo_ctx* c, int precision) {
     if (precision != PHYLO_PREC_FP32 && precision != PHYLO_PREC_FP64) return fail(c, PHYLO_ERR_ARG, "precision");
     if (precision != c->prec) {
         c->prec = precision;
+        c->clv_valid = false;
         if (c->d_clv) { (void)hipFree(c->d_clv); c->d_clv = nullptr; c->clv_bytes = 0; }
     }
     return PHYLO_OK;
@@ -895,6 +1034,7 @@ int phylo_set_tips(phylo_ctx* c, int n_taxa, int n_sites, int n_states, const ui
     for (auto& x : c->masks) x &= valid;
     c->have_tips = true;
     c->dirty_tips = true;
+    c->clv_valid = false;
     c->dirty_roots = true;
     c->dirty_topo = true;     // slot numbering depends on n_taxa
     c->have_site_w = false;
@@ -938,6 +1078,7 @@ int phylo_set_topology(phylo_ctx* c, int n_nodes, int root, const int32_t* left,
     c->right.assign(right, right + n_nodes);
     c->have_topo = true;
     c->dirty_topo = true;
+    c->clv_valid = false;
     c->have_eff = false;
     c->dirty_P = true;
     return PHYLO_OK;
@@ -953,6 +1094,7 @@ int phylo_set_edge_lengths(phylo_ctx* c, const double* eff) {
     c->eff[c->root] = 0.0;
     c->have_eff = true;
     c->dirty_P = true;
+    c->clv_valid = false;
     return PHYLO_OK;
 }
 
@@ -978,6 +1120,7 @@ int phylo_set_model(phylo_ctx* c, int n_states, const double* Q, const double* r
     }
     c->have_model = true;
     c->dirty_P = true;
+    c->clv_valid = false;
     c->dirty_roots = true;
     return PHYLO_OK;
 }
@@ -992,6 +1135,7 @@ int phylo_set_categories(phylo_ctx* c, int n_cat, const double* rates, const dou
     c->cat_rate.assign(rates, rates + n_cat);
     c->cat_w.assign(weights, weights + n_cat);
     c->dirty_P = true;
+    c->clv_valid = false;
     c->dirty_roots = true;
     if (c->d_clv) { (void)hipFree(c->d_clv); c->d_clv = nullptr; c->clv_bytes = 0; }
     return PHYLO_OK;
@@ -1002,7 +1146,12 @@ int phylo_loglik(phylo_ctx* c, double* site_lnl, double* total) {
     if (!total) return fail(c, PHYLO_ERR_ARG, "loglik: total is required");
     int rc = prepare(c);
     if (rc) return rc;
-    return run_programs(c, 1, total, site_lnl);
+    c->clv_valid = false;
+    rc = run_programs(c, c->d_ops, c->d_prog, 1, total, site_lnl);
+    if (rc) return rc;
+    c->clv_valid = true;
+    c->base_total = *total;
+    return PHYLO_OK;
 }
 
 int phylo_node_partials(phylo_ctx* c, int node, double* clv, int32_t* scale) {
@@ -1051,6 +1200,57 @@ int phylo_get_pmatrix(phylo_ctx* c, int node, double* P) {
     for (int k = 0; k < c->C; ++k)
         for (int i = 0; i < S; ++i)
             for (int j = 0; j < S; ++j) P[((size_t)k * S + i) * S + j] = buf[((size_t)k * Sp + i) * Sp + j];
+    return PHYLO_OK;
+}
+
+int phylo_nni_score(phylo_ctx* c, int mode, int max_variants, double* lnl, int32_t* node, int32_t* side,
+                    int* n_variants, double* base_lnl) {
+    if (!c || !n_variants) return PHYLO_ERR_ARG;
+    if (mode != 0 && mode != 1) return fail(c, PHYLO_ERR_ARG, "nni_score: mode must be 0 or 1");
+    int rc = prepare(c);
+    if (rc) return rc;
+    double base = 0.0;
+    if (!c->clv_valid) {
+        rc = phylo_loglik(c, nullptr, &base);
+        if (rc) return rc;
+    }
+    base = c->base_total;
+    std::vector<int> nodes, sides;
+    enumerate_nni(c, mode, nodes, sides);
+    const int nv = (int)nodes.size();
+    *n_variants = nv;
+    if (base_lnl) *base_lnl = base;
+    if (nv == 0 || !lnl) return PHYLO_OK;   // count query
+    if (max_variants < nv) return fail(c, PHYLO_ERR_ARG, "nni_score: output arrays too small");
+    const std::vector<int> par = parents_of(c);
+    std::vector<Op> ops;
+    std::vector<int> begin{0}, which;
+    for (int k = 0; k < nv; ++k) {
+        if (append_nni_program(c, par, nodes[k], sides[k], ops)) {
+            begin.push_back((int)ops.size());
+            which.push_back(k);
+        }
+        lnl[k] = base;   // no-op interchanges keep the base tree
+        if (node) node[k] = nodes[k];
+        if (side) side[k] = sides[k];
+    }
+    const int n_prog = (int)which.size();
+    if (n_prog == 0) return PHYLO_OK;
+    HIPCHK(c, ensure(c->d_nni_ops, c->nni_ops_bytes, ops.size() * sizeof(Op)));
+    HIPCHK(c, ensure(c->d_nni_prog, c->nni_prog_bytes, begin.size() * sizeof(int)));
+    HIPCHK(c, hipMemcpyAsync(c->d_nni_ops, ops.data(), ops.size() * sizeof(Op), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_nni_prog, begin.data(), begin.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    std::vector<double> tot(n_prog);
+    rc = run_programs(c, c->d_nni_ops, c->d_nni_prog, n_prog, tot.data(), nullptr);
+    if (rc) return rc;
+    for (int q = 0; q < n_prog; ++q) lnl[which[q]] = tot[q];
+    c->nni_ops_last = (long)ops.size();
+    return PHYLO_OK;
+}
+
+int phylo_nni_ops(phylo_ctx* c, long* n_ops) {
+    if (!c || !n_ops) return PHYLO_ERR_ARG;
+    *n_ops = c->nni_ops_last;
     return PHYLO_OK;
 }
 
