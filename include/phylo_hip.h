@@ -137,6 +137,47 @@ int phylo_synchronize(phylo_ctx *ctx);
  * kernel in the last phylo_loglik / phylo_nni_score call. */
 int phylo_last_kernel_ms(phylo_ctx *ctx, double *ms);
 
+/* ---- Batched scoring ------------------------------------------------------
+ * Many independent DNA problems (S <= 4) of uneven taxon and site counts,
+ * each optionally with its whole NNI neighbourhood, scored by two persistent
+ * work-queue launches (full evaluations, then partial-recompute variants)
+ * captured once into a HIP graph and replayed by phylo_batch_run.  All
+ * problems share the number of rate categories and the precision.
+ * Replaces: running treeator once per tree file / per treebender --nni tree
+ * (treeator.cpp:705-990), which has no batching. */
+typedef struct phylo_batch phylo_batch;
+
+int phylo_batch_create(int device, int n_cat, int precision, phylo_batch **out);
+const char *phylo_batch_last_error(const phylo_batch *b);
+
+/* Adds one problem (copied): tip masks [n_taxa][n_sites], topology as in
+ * phylo_set_topology, effective edge lengths, a reversible Q with its pi
+ * (P(t) from the eigen-system), root weights, category rates and weights,
+ * nni_mode -1 (none), 0 or 1 (as phylo_nni_score).  *index: its position. */
+int phylo_batch_add(phylo_batch *b, int n_taxa, int n_sites, int n_states, const uint32_t *masks, int n_nodes,
+                    int root, const int32_t *left, const int32_t *right, const double *eff_len, const double *Q,
+                    const double *root_weights, const double *pi, const double *cat_rates,
+                    const double *cat_weights, int nni_mode, int *index);
+
+/* Uploads everything, builds the work lists and captures the graph.  With
+ * autotune != 0 it first times 64, 128, 256 and 512 sites per work item and
+ * keeps the fastest. */
+int phylo_batch_finalize(phylo_batch *b, int autotune);
+
+/* Replays the graph.  totals: one lnL per problem; nni_lnl (nullable): every
+ * problem's variants, concatenated in phylo_batch_nni_layout order. */
+int phylo_batch_run(phylo_batch *b, double *totals, double *nni_lnl);
+
+/* offsets [n_problems + 1] into the variant list; node/side per variant. */
+int phylo_batch_nni_layout(phylo_batch *b, int32_t *offsets, int32_t *node, int32_t *side);
+
+/* Updates (site x node x category) per run in each phase, device time of the
+ * last run (ms), the chosen sites per work item and the variant count. */
+int phylo_batch_stats(phylo_batch *b, double *updates_full, double *updates_nni, double *last_ms,
+                      int *sites_per_item, int *n_variants);
+
+void phylo_batch_destroy(phylo_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,17 @@ SIGNATURES = [
                                  ctypes.POINTER(ctypes.c_int), _c_dbl_p]),
     ("phylo_nni_ops", _c_int, [_c_void_p, ctypes.POINTER(ctypes.c_long)]),
     ("phylo_synchronize", _c_int, [_c_void_p]),
+    ("phylo_batch_create", _c_int, [_c_int, _c_int, _c_int, ctypes.POINTER(_c_void_p)]),
+    ("phylo_batch_last_error", ctypes.c_char_p, [_c_void_p]),
+    ("phylo_batch_add", _c_int, [_c_void_p, _c_int, _c_int, _c_int, ctypes.POINTER(ctypes.c_uint32), _c_int, _c_int,
+                                 _c_int_p, _c_int_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p,
+                                 _c_int, ctypes.POINTER(ctypes.c_int)]),
+    ("phylo_batch_finalize", _c_int, [_c_void_p, _c_int]),
+    ("phylo_batch_run", _c_int, [_c_void_p, _c_dbl_p, _c_dbl_p]),
+    ("phylo_batch_nni_layout", _c_int, [_c_void_p, _c_int_p, _c_int_p, _c_int_p]),
+    ("phylo_batch_stats", _c_int, [_c_void_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_int)]),
+    ("phylo_batch_destroy", None, [_c_void_p]),
     ("phylo_last_kernel_ms", _c_int, [_c_void_p, _c_dbl_p]),
 ]
 
@@ -203,3 +214,81 @@ class Context:
         ms = np.zeros(1)
         self._check(self.lib.phylo_last_kernel_ms(self.h, _dp(ms)))
         return float(ms[0])
+
+
+class Batch:
+    """Many independent DNA problems (and their NNI neighbourhoods) scored by
+    persistent work-queue launches replayed from one captured HIP graph."""
+
+    def __init__(self, n_cat: int = 4, precision: int = PREC_FP32, device: int = 0):
+        self.lib = load_library()
+        h = _c_void_p()
+        rc = self.lib.phylo_batch_create(device, n_cat, precision, ctypes.byref(h))
+        if rc != PHYLO_OK:
+            raise PhyloError(rc, "phylo_batch_create failed")
+        self.h = h
+        self.n_cat = n_cat
+        self.n = 0
+        self.n_var = 0
+
+    def _check(self, rc):
+        if rc != PHYLO_OK:
+            raise PhyloError(rc, self.lib.phylo_batch_last_error(self.h).decode())
+
+    def add(self, masks, left, right, root, eff, Q, pi, cat_rates, cat_w=None, nni_mode=-1, root_w=None):
+        masks = np.ascontiguousarray(masks, dtype=np.uint32)
+        left = np.ascontiguousarray(left, dtype=np.int32)
+        right = np.ascontiguousarray(right, dtype=np.int32)
+        eff = np.ascontiguousarray(eff, dtype=np.float64)
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        pi = np.ascontiguousarray(pi, dtype=np.float64)
+        rw = pi if root_w is None else np.ascontiguousarray(root_w, dtype=np.float64)
+        cr = np.ascontiguousarray(cat_rates, dtype=np.float64)
+        cw = np.full(len(cr), 1.0 / len(cr)) if cat_w is None else np.ascontiguousarray(cat_w, dtype=np.float64)
+        idx = ctypes.c_int(-1)
+        self._check(self.lib.phylo_batch_add(self.h, masks.shape[0], masks.shape[1], Q.shape[0],
+                                             masks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(left),
+                                             int(root), _ip(left), _ip(right), _dp(eff), _dp(Q), _dp(rw), _dp(pi),
+                                             _dp(cr), _dp(cw), nni_mode, ctypes.byref(idx)))
+        self.n += 1
+        return idx.value
+
+    def finalize(self, autotune: bool = True):
+        self._check(self.lib.phylo_batch_finalize(self.h, int(autotune)))
+        _, _, _, _, self.n_var = self.stats()
+
+    def run(self):
+        tot = np.zeros(max(self.n, 1))
+        nni = np.zeros(max(self.n_var, 1))
+        self._check(self.lib.phylo_batch_run(self.h, _dp(tot), _dp(nni)))
+        return tot[: self.n], nni[: self.n_var]
+
+    def nni_layout(self):
+        off = np.zeros(self.n + 1, np.int32)
+        node = np.zeros(max(self.n_var, 1), np.int32)
+        side = np.zeros(max(self.n_var, 1), np.int32)
+        self._check(self.lib.phylo_batch_nni_layout(self.h, _ip(off), _ip(node), _ip(side)))
+        return off, node[: self.n_var], side[: self.n_var]
+
+    def stats(self):
+        u1, u2, ms = np.zeros(1), np.zeros(1), np.zeros(1)
+        spi, nv = ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self.lib.phylo_batch_stats(self.h, _dp(u1), _dp(u2), _dp(ms), ctypes.byref(spi), ctypes.byref(nv)))
+        return float(u1[0]), float(u2[0]), float(ms[0]), spi.value, nv.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.phylo_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -110,20 +110,12 @@ __device__ inline void block_sum_store(double v, double* dst) {
 // site share one power-of-two scaler: each wave proposes the frexp exponent of
 // its per-site maximum, the proposals meet in LDS (double-buffered, one
 // barrier per op) and every wave applies the largest.
-// grid = (m_pad / 64, n_programs); program y runs ops[prog_begin[y] .. prog_begin[y+1]).
+// walk4 runs ops[k0, k1) for site s and returns the site's lnL (root op).
 template <typename T, int C>
-__global__ __launch_bounds__(64 * C) void prune4_kernel(
-    const Op* __restrict__ ops, const int* __restrict__ prog_begin, const uint8_t* __restrict__ tips,
-    T* clv, int* scl, const double* __restrict__ P, const double* __restrict__ roots,
-    const double* __restrict__ site_w, double* __restrict__ site_lnl, double* __restrict__ partial,
-    int m, int m_pad) {
-    __shared__ int xexp[2][C][64];
-    __shared__ double xroot[C][64];
-    const int lane = threadIdx.x & 63;
-    const int cat = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int s = blockIdx.x * 64 + lane;
-    const int prog = blockIdx.y;
-    const int k0 = prog_begin[prog], k1 = prog_begin[prog + 1];
+__device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const uint8_t* __restrict__ tips,
+                               T* clv, int* scl, const double* __restrict__ P, const double* __restrict__ roots,
+                               int m_pad, int s, int lane, int cat, int (&xexp)[2][C][64],
+                               double (&xroot)[C][64]) {
     T cA[4] = {T(0), T(0), T(0), T(0)}, cB[4] = {T(0), T(0), T(0), T(0)};
     int sA = 0, sB = 0;
     double lnl = 0.0;
@@ -238,10 +230,93 @@ __global__ __launch_bounds__(64 * C) void prune4_kernel(
             lnl = log(lc) + sc * kLn2;
         }
     }
+    return lnl;
+}
+
+// grid = (m_pad / 64, n_programs); program y runs ops[prog_begin[y] .. prog_begin[y+1]).
+template <typename T, int C>
+__global__ __launch_bounds__(64 * C) void prune4_kernel(
+    const Op* __restrict__ ops, const int* __restrict__ prog_begin, const uint8_t* __restrict__ tips,
+    T* clv, int* scl, const double* __restrict__ P, const double* __restrict__ roots,
+    const double* __restrict__ site_w, double* __restrict__ site_lnl, double* __restrict__ partial,
+    int m, int m_pad) {
+    __shared__ int xexp[2][C][64];
+    __shared__ double xroot[C][64];
+    const int lane = threadIdx.x & 63;
+    const int cat = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int s = blockIdx.x * 64 + lane;
+    const int prog = blockIdx.y;
+    const double lnl = walk4<T, C>(ops, prog_begin[prog], prog_begin[prog + 1], tips, clv, scl, P, roots, m_pad, s,
+                                   lane, cat, xexp, xroot);
     const bool live = s < m && cat == 0;
     if (live && site_lnl) site_lnl[(size_t)prog * m + s] = lnl;
     const double w = live ? (site_w ? site_w[s] : 1.0) : 0.0;
     block_sum_store(live ? w * lnl : 0.0, partial + (size_t)prog * gridDim.x + blockIdx.x);
+}
+
+// Batched scoring: many independent problems (and their NNI programs) in one
+// persistent launch.  Workgroup g takes work items (instance, program, block
+// of 64 sites) g, g + grid, g + 2*grid, ... of a list sorted by decreasing
+// cost; the loop counter is uniform, so every wave runs the same barriers and
+// every workgroup reaches the exit.  Each item stores its partial sum at
+// items[].out.  (A thread-0 atomic work queue is deliberately not used: the
+// compiler merges a thread-0 queue fetch at the top of the loop with the
+// thread-0 partial store at its tail into one divergent region, and the other
+// lanes then loop on a stale item.)
+struct BatchInst {
+    long tips_off;    // into the uint8 tip-code buffer
+    long clv_off;     // elements into the partial buffer
+    long scl_off;     // into the scaler buffer
+    long P_off;       // doubles into the P buffer
+    int roots_off;    // doubles into the root-weight buffer
+    int m, m_pad, pad;
+};
+struct WorkItem {
+    int inst, prog, block, out;
+};
+
+template <typename T, int C>
+__global__ __launch_bounds__(64 * C) void batch4_kernel(const WorkItem* __restrict__ items, int n_items,
+                                                        const BatchInst* __restrict__ inst,
+                                                        const int* __restrict__ prog_begin,
+                                                        const Op* __restrict__ ops, const uint8_t* __restrict__ tips,
+                                                        T* clv, int* scl, const double* __restrict__ P,
+                                                        const double* __restrict__ roots,
+                                                        double* __restrict__ partial, int blocks_per_item) {
+    __shared__ int xexp[2][C][64];
+    __shared__ double xroot[C][64];
+    const int lane = threadIdx.x & 63;
+    const int cat = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const WorkItem w = items[it];
+        const BatchInst I = inst[w.inst];
+        const int k0 = prog_begin[w.prog], k1 = prog_begin[w.prog + 1];
+        double acc = 0.0;
+        for (int q = 0; q < blocks_per_item; ++q) {
+            const int s = (w.block + q) * 64 + lane;
+            if ((w.block + q) * 64 >= I.m_pad) break;
+            const double lnl = walk4<T, C>(ops, k0, k1, tips + I.tips_off, clv + I.clv_off, scl + I.scl_off,
+                                           P + I.P_off, roots + I.roots_off, I.m_pad, s, lane, cat, xexp, xroot);
+            if (s < I.m && cat == 0) acc += lnl;
+        }
+        block_sum_store(acc, partial + w.out);
+    }
+}
+
+// Segmented compensated sums: out[k] = sum partial[seg[k] .. seg[k+1]).
+__global__ __launch_bounds__(64) void segment_sum_kernel(const double* __restrict__ partial,
+                                                        const int* __restrict__ seg, int n_seg,
+                                                        double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_seg) return;
+    double s = 0.0, comp = 0.0;
+    for (int i = seg[k]; i < seg[k + 1]; ++i) {
+        const double x = partial[i] - comp;
+        const double t = s + x;
+        comp = (t - s) - x;
+        s = t;
+    }
+    out[k] = s;
 }
 
 // One child of the wide-state walker: acc[i] (=|*=) sum_j P[i][j] x_j.
@@ -566,9 +641,11 @@ int padded_states(int S) {
     return 32;
 }
 
-// Builds the full-evaluation program: internal nodes in the reference's
-// postorder, carry A holding the previous op's result.
-int compile_full_program(phylo_ctx* c) {
+// Appends the full-evaluation program of a tree: internal nodes in the
+// reference's postorder (left subtree, right subtree, node), carry A holding
+// the previous op's result.  Returns false if internal nodes are unreachable.
+template <class TV>
+bool build_full_ops(const TV* c, std::vector<Op>& ops) {
     const int n = c->n_nodes;
     std::vector<int> order;
     order.reserve(n);
@@ -585,9 +662,7 @@ int compile_full_program(phylo_ctx* c) {
             if (v >= c->n_taxa) order.push_back(v);
         }
     }
-    if ((int)order.size() != n - c->n_taxa)
-        return fail(c, PHYLO_ERR_ARG, "topology: internal nodes unreachable from the root");
-    c->ops.clear();
+    if ((int)order.size() != n - c->n_taxa) return false;
     int prev = -1;
     auto src = [&](int child, int& kind, int& idx) {
         if (child < 0) { kind = SRC_NONE; idx = 0; }
@@ -603,11 +678,17 @@ int compile_full_program(phylo_ctx* c) {
         src(c->right[v], op.r_kind, op.r_idx);
         op.out_slot = v - c->n_taxa;
         op.flags = (v == c->root) ? OP_ROOT : 0;
-        c->ops.push_back(op);
+        ops.push_back(op);
         prev = v;
     }
+    return true;
+}
+
+int compile_full_program(phylo_ctx* c) {
+    c->ops.clear();
+    if (!build_full_ops(c, c->ops)) return fail(c, PHYLO_ERR_ARG, "topology: internal nodes unreachable from the root");
     c->prog_begin = {0, (int)c->ops.size()};
-    c->n_slots = n - c->n_taxa;
+    c->n_slots = c->n_nodes - c->n_taxa;
     return PHYLO_OK;
 }
 
@@ -835,7 +916,8 @@ int run_programs(phylo_ctx* c, const Op* d_ops, const int* d_prog, int n_prog, d
 // NNI neighbourhoods (tree.cpp:2468-2548, treebender.cpp:897-915)
 // ---------------------------------------------------------------------------
 
-std::vector<int> parents_of(const phylo_ctx* c) {
+template <class TV>
+std::vector<int> parents_of(const TV* c) {
     std::vector<int> par(c->n_nodes, -1);
     for (int v = 0; v < c->n_nodes; ++v) {
         if (c->left[v] >= 0) par[c->left[v]] = v;
@@ -844,7 +926,8 @@ std::vector<int> parents_of(const phylo_ctx* c) {
     return par;
 }
 
-int count_tips(const phylo_ctx* c, int v) {
+template <class TV>
+int count_tips(const TV* c, int v) {
     if (v < 0) return 0;
     if (c->left[v] < 0 && c->right[v] < 0) return 1;
     return count_tips(c, c->left[v]) + count_tips(c, c->right[v]);
@@ -854,7 +937,8 @@ int count_tips(const phylo_ctx* c, int v) {
 // branches 2..n-3 numbered in preorder by get_internal_branch_by_number_unrooted,
 // tree.cpp:2455-2478; left then right swap); mode 1 is every internal edge of
 // the unrooted tree, both root children counting as one edge.
-void enumerate_nni(const phylo_ctx* c, int mode, std::vector<int>& nodes, std::vector<int>& sides) {
+template <class TV>
+void enumerate_nni(const TV* c, int mode, std::vector<int>& nodes, std::vector<int>& sides) {
     nodes.clear();
     sides.clear();
     const int r = c->root;
@@ -894,7 +978,8 @@ void enumerate_nni(const phylo_ctx* c, int mode, std::vector<int>& nodes, std::v
 // (tree.cpp:2480-2518): the two nodes whose children change, then the path to
 // the root, each child coming from a stored partial, a tip or a register
 // carry.  Returns false when the interchange is a no-op (no swap partner).
-bool append_nni_program(const phylo_ctx* c, const std::vector<int>& par, int v, int side, std::vector<Op>& ops) {
+template <class TV>
+bool append_nni_program(const TV* c, const std::vector<int>& par, int v, int side, std::vector<Op>& ops) {
     if (v < 0 || v == c->root || c->left[v] < 0) return false;
     const int p = par[v], r = c->root;
     const auto mem = [&](int x, int& kind, int& idx) {
@@ -1265,6 +1350,474 @@ int phylo_last_kernel_ms(phylo_ctx* c, double* ms) {
     if (!c || !ms) return PHYLO_ERR_ARG;
     *ms = c->last_ms;
     return PHYLO_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched scoring of many independent problems (+ NNI neighbourhoods)
+// ---------------------------------------------------------------------------
+
+struct phylo_batch {
+    struct Inst {
+        int n_taxa = 0, n_nodes = 0, root = -1, m = 0, m_pad = 0, S = 4, nni_mode = -1;
+        std::vector<int> left, right;
+        std::vector<uint32_t> masks;
+        std::vector<double> P;      // [n_nodes][C][16]
+        double roots[kMaxStates + kMaxCat];
+        long tips_off = 0, clv_off = 0, scl_off = 0, P_off = 0;
+        int roots_off = 0;
+        int full_prog = -1;
+        std::vector<int> nni_prog, nni_node, nni_side;   // nni_prog[k] = -1: no-op variant
+        long full_ops = 0, nni_ops = 0;
+    };
+    int device = 0;
+    int C = 4, prec = PHYLO_PREC_FP32;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    std::vector<Inst> inst;
+    bool finalized = false;
+    int blocks_per_item = 1, grid = 0, n_items1 = 0, n_items2 = 0, n_seg1 = 0, n_seg2 = 0, n_var = 0;
+    double updates1 = 0, updates2 = 0, last_ms = 0;
+    std::vector<int> var_offsets;     // per instance into the flat variant list
+    // device
+    uint8_t* d_tips = nullptr;
+    void* d_clv = nullptr;
+    int* d_scl = nullptr;
+    double* d_P = nullptr;
+    double* d_roots = nullptr;
+    Op* d_ops = nullptr;
+    int* d_prog = nullptr;
+    BatchInst* d_inst = nullptr;
+    WorkItem* d_items1 = nullptr;
+    WorkItem* d_items2 = nullptr;
+    double* d_partial = nullptr;
+    int* d_seg1 = nullptr;
+    int* d_seg2 = nullptr;
+    double* d_out = nullptr;
+    double* h_out = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    // host copies used to build work lists
+    std::vector<WorkItem> items1, items2;
+    std::vector<int> seg1, seg2;
+    std::vector<int> var_prog_of;     // flat variant -> program (-1 = no-op)
+};
+
+namespace {
+
+int fail(phylo_batch* b, int code, const std::string& msg) {
+    if (b) b->err = msg;
+    return code;
+}
+
+template <typename T, int C>
+void launch_batch(phylo_batch* b, const WorkItem* items, int n_items, int bpi) {
+    hipLaunchKernelGGL((batch4_kernel<T, C>), dim3(b->grid), dim3(64 * C), 0, b->stream, items, n_items, b->d_inst, b->d_prog, b->d_ops, b->d_tips, reinterpret_cast<T*>(b->d_clv), b->d_scl, b->d_P,
+                       b->d_roots, b->d_partial, bpi);
+}
+
+int enqueue_batch_phase(phylo_batch* b, int phase, int bpi) {
+    const WorkItem* items = phase == 1 ? b->d_items1 : b->d_items2;
+    const int n = phase == 1 ? b->n_items1 : b->n_items2;
+    if (n == 0) return PHYLO_OK;
+    const bool f64 = b->prec == PHYLO_PREC_FP64;
+    switch (b->C) {
+    case 1: f64 ? launch_batch<double, 1>(b, items, n, bpi) : launch_batch<float, 1>(b, items, n, bpi); break;
+    case 2: f64 ? launch_batch<double, 2>(b, items, n, bpi) : launch_batch<float, 2>(b, items, n, bpi); break;
+    case 4: f64 ? launch_batch<double, 4>(b, items, n, bpi) : launch_batch<float, 4>(b, items, n, bpi); break;
+    default: f64 ? launch_batch<double, 8>(b, items, n, bpi) : launch_batch<float, 8>(b, items, n, bpi); break;
+    }
+    return PHYLO_OK;
+}
+
+// Work lists for a given number of 64-site blocks per item.  Items are
+// ordered by decreasing estimated cost (program length x sites) so the
+// round-robin assignment to workgroups finishes evenly; partial sums are
+// placed by segment.
+void build_work_lists(phylo_batch* b, int bpi, const std::vector<int>& prog_len) {
+    struct Cand { long cost; WorkItem w; };
+    std::vector<Cand> c1, c2;
+    b->seg1.assign(1, 0);
+    b->seg2.assign(1, 0);
+    int out = 0;
+    for (size_t i = 0; i < b->inst.size(); ++i) {
+        const auto& I = b->inst[i];
+        const int nb = I.m_pad / 64;
+        for (int blk = 0; blk < nb; blk += bpi) {
+            const int sites = std::min(bpi, nb - blk) * 64;
+            c1.push_back({(long)prog_len[I.full_prog] * sites, {(int)i, I.full_prog, blk, out++}});
+        }
+        b->seg1.push_back(out);
+    }
+    const int base2 = out;
+    for (size_t i = 0; i < b->inst.size(); ++i) {
+        const auto& I = b->inst[i];
+        const int nb = I.m_pad / 64;
+        for (int pg : I.nni_prog) {
+            if (pg < 0) continue;
+            for (int blk = 0; blk < nb; blk += bpi) {
+                const int sites = std::min(bpi, nb - blk) * 64;
+                c2.push_back({(long)prog_len[pg] * sites, {(int)i, pg, blk, out++}});
+            }
+            b->seg2.push_back(out - base2);
+        }
+    }
+    auto by_cost = [](const Cand& x, const Cand& y) { return x.cost > y.cost; };
+    std::stable_sort(c1.begin(), c1.end(), by_cost);
+    std::stable_sort(c2.begin(), c2.end(), by_cost);
+    b->items1.clear();
+    b->items2.clear();
+    for (auto& x : c1) b->items1.push_back(x.w);
+    for (auto& x : c2) b->items2.push_back(x.w);
+    for (auto& s : b->seg2) s += base2;
+    b->seg2.front() = base2;
+}
+
+int upload_work_lists(phylo_batch* b) {
+    b->n_items1 = (int)b->items1.size();
+    b->n_items2 = (int)b->items2.size();
+    b->n_seg1 = (int)b->seg1.size() - 1;
+    b->n_seg2 = (int)b->seg2.size() - 1;
+    auto up = [&](auto*& dptr, const auto& vec) -> hipError_t {
+        if (dptr) { hipError_t e = hipFree(dptr); if (e != hipSuccess) return e; dptr = nullptr; }
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&dptr), std::max<size_t>(vec.size(), 1) * sizeof(vec[0]));
+        if (e != hipSuccess) return e;
+        if (vec.empty()) return hipSuccess;
+        return hipMemcpy(dptr, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice);
+    };
+    HIPCHK(b, up(b->d_items1, b->items1));
+    HIPCHK(b, up(b->d_items2, b->items2));
+    HIPCHK(b, up(b->d_seg1, b->seg1));
+    HIPCHK(b, up(b->d_seg2, b->seg2));
+    if (b->d_partial) HIPCHK(b, hipFree(b->d_partial));
+    b->d_partial = nullptr;
+    HIPCHK(b, hipMalloc(&b->d_partial, std::max<size_t>(b->n_items1 + b->n_items2, 1) * sizeof(double)));
+    return PHYLO_OK;
+}
+
+// Enqueues one full batch run (both phases and the segment sums) on the stream.
+int enqueue_batch_run(phylo_batch* b) {
+    int rc = enqueue_batch_phase(b, 1, b->blocks_per_item);
+    if (rc) return rc;
+    rc = enqueue_batch_phase(b, 2, b->blocks_per_item);
+    if (rc) return rc;
+    if (b->n_seg1)
+        hipLaunchKernelGGL(segment_sum_kernel, dim3((b->n_seg1 + 63) / 64), dim3(64), 0, b->stream, b->d_partial,
+                           b->d_seg1, b->n_seg1, b->d_out);
+    if (b->n_seg2)
+        hipLaunchKernelGGL(segment_sum_kernel, dim3((b->n_seg2 + 63) / 64), dim3(64), 0, b->stream, b->d_partial,
+                           b->d_seg2, b->n_seg2, b->d_out + b->n_seg1);
+    HIPCHK(b, hipGetLastError());
+    HIPCHK(b, hipMemcpyAsync(b->h_out, b->d_out, (size_t)(b->n_seg1 + b->n_seg2) * sizeof(double),
+                             hipMemcpyDeviceToHost, b->stream));
+    return PHYLO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int phylo_batch_create(int device, int n_cat, int precision, phylo_batch** out) {
+    if (!out) return PHYLO_ERR_ARG;
+    *out = nullptr;
+    if (n_cat != 1 && n_cat != 2 && n_cat != 4 && n_cat != 8) return PHYLO_ERR_ARG;
+    if (precision != PHYLO_PREC_FP32 && precision != PHYLO_PREC_FP64) return PHYLO_ERR_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return PHYLO_ERR_HIP;
+    if (device < 0 || device >= n) return PHYLO_ERR_ARG;
+    phylo_batch* b = new phylo_batch();
+    b->device = device;
+    b->C = n_cat;
+    b->prec = precision;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        delete b;
+        return PHYLO_ERR_HIP;
+    }
+    *out = b;
+    return PHYLO_OK;
+}
+
+const char* phylo_batch_last_error(const phylo_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int phylo_batch_add(phylo_batch* b, int n_taxa, int n_sites, int n_states, const uint32_t* masks, int n_nodes, int root,
+                    const int32_t* left, const int32_t* right, const double* eff, const double* Q,
+                    const double* root_w, const double* pi, const double* cat_rates, const double* cat_w,
+                    int nni_mode, int* index) {
+    if (!b) return PHYLO_ERR_ARG;
+    if (b->finalized) return fail(b, PHYLO_ERR_STATE, "batch_add after batch_finalize");
+    if (n_states < 1 || n_states > 4) return fail(b, PHYLO_ERR_UNSUPPORTED, "batch_add: batches hold up to 4 states");
+    if (n_taxa < 2 || n_sites < 1 || n_nodes <= n_taxa || root < n_taxa || root >= n_nodes || !masks || !left ||
+        !right || !eff || !Q || !root_w || !pi || !cat_rates || !cat_w || nni_mode < -1 || nni_mode > 1)
+        return fail(b, PHYLO_ERR_ARG, "batch_add: bad arguments");
+    phylo_batch::Inst I;
+    I.n_taxa = n_taxa;
+    I.m = n_sites;
+    I.m_pad = (n_sites + 63) / 64 * 64;
+    I.S = n_states;
+    I.n_nodes = n_nodes;
+    I.root = root;
+    I.left.assign(left, left + n_nodes);
+    I.right.assign(right, right + n_nodes);
+    std::vector<int> indeg(n_nodes, 0);
+    for (int v = 0; v < n_nodes; ++v) {
+        if (v < n_taxa && (left[v] != -1 || right[v] != -1)) return fail(b, PHYLO_ERR_ARG, "batch_add: tip with children");
+        if (v >= n_taxa) {
+            if (left[v] < 0 || left[v] >= n_nodes || right[v] < -1 || right[v] >= n_nodes)
+                return fail(b, PHYLO_ERR_ARG, "batch_add: bad child index");
+            ++indeg[left[v]];
+            if (right[v] >= 0) ++indeg[right[v]];
+        }
+    }
+    for (int v = 0; v < n_nodes; ++v)
+        if ((v == root) != (indeg[v] == 0) || indeg[v] > 1) return fail(b, PHYLO_ERR_ARG, "batch_add: not a tree");
+    const uint32_t valid = (1u << n_states) - 1u;
+    I.masks.assign(masks, masks + (size_t)n_taxa * n_sites);
+    for (auto& x : I.masks) {
+        x &= valid;
+        if (!x) return fail(b, PHYLO_ERR_DATA, "batch_add: a tip lacks a character state");
+    }
+    // P per (node, category) from the reversible eigen-system, on the host
+    std::vector<double> V, lam, Vinv;
+    if (!phylo_math::is_reversible(n_states, Q, pi) || !phylo_math::reversible_eigen(n_states, Q, pi, V, lam, Vinv))
+        return fail(b, PHYLO_ERR_ARG, "batch_add: Q must be reversible with positive pi");
+    I.P.assign((size_t)n_nodes * b->C * 16, 0.0);
+    for (int v = 0; v < n_nodes; ++v) {
+        if (v == root) continue;
+        if (!std::isfinite(eff[v])) return fail(b, PHYLO_ERR_ARG, "batch_add: non-finite branch length");
+        for (int k = 0; k < b->C; ++k) {
+            const double t = eff[v] * cat_rates[k];
+            double* dst = I.P.data() + ((size_t)v * b->C + k) * 16;
+            for (int i = 0; i < n_states; ++i)
+                for (int j = 0; j < n_states; ++j) {
+                    double a = 0.0;
+                    for (int q = 0; q < n_states; ++q)
+                        a += V[(size_t)i * n_states + q] * std::exp(lam[q] * t) * Vinv[(size_t)q * n_states + j];
+                    dst[i * 4 + j] = a;
+                }
+        }
+    }
+    for (double& r : I.roots) r = 0.0;
+    for (int i = 0; i < n_states; ++i) I.roots[i] = root_w[i];
+    for (int k = 0; k < b->C; ++k) I.roots[kRootCatOff + k] = cat_w[k];
+    I.nni_mode = nni_mode;
+    b->inst.push_back(std::move(I));
+    if (index) *index = (int)b->inst.size() - 1;
+    return PHYLO_OK;
+}
+
+int phylo_batch_finalize(phylo_batch* b, int autotune) {
+    if (!b) return PHYLO_ERR_ARG;
+    if (b->finalized) return fail(b, PHYLO_ERR_STATE, "batch already finalized");
+    if (b->inst.empty()) return fail(b, PHYLO_ERR_STATE, "empty batch");
+    HIPCHK(b, hipSetDevice(b->device));
+    std::vector<Op> ops;
+    std::vector<int> prog_begin{0}, prog_len;
+    long tips_bytes = 0, clv_elems = 0, scl_elems = 0, P_elems = 0;
+    int roots_elems = 0;
+    b->updates1 = b->updates2 = 0;
+    b->var_offsets.assign(1, 0);
+    b->var_prog_of.clear();
+    for (auto& I : b->inst) {
+        I.tips_off = tips_bytes;
+        tips_bytes += (long)I.n_taxa * I.m_pad;
+        I.clv_off = clv_elems;
+        clv_elems += (long)(I.n_nodes - I.n_taxa) * b->C * I.m_pad * 4;
+        I.scl_off = scl_elems;
+        scl_elems += (long)(I.n_nodes - I.n_taxa) * I.m_pad;
+        I.P_off = P_elems;
+        P_elems += (long)I.n_nodes * b->C * 16;
+        I.roots_off = roots_elems;
+        roots_elems += kMaxStates + kMaxCat;
+        const size_t before = ops.size();
+        if (!build_full_ops(&I, ops)) return fail(b, PHYLO_ERR_ARG, "batch: internal nodes unreachable from the root");
+        I.full_prog = (int)prog_begin.size() - 1;
+        prog_begin.push_back((int)ops.size());
+        prog_len.push_back((int)(ops.size() - before));
+        I.full_ops = (long)(ops.size() - before);
+        b->updates1 += (double)I.full_ops * I.m * b->C;
+        I.nni_prog.clear();
+        I.nni_node.clear();
+        I.nni_side.clear();
+        I.nni_ops = 0;
+        if (I.nni_mode >= 0) {
+            const std::vector<int> par = parents_of(&I);
+            enumerate_nni(&I, I.nni_mode, I.nni_node, I.nni_side);
+            for (size_t k = 0; k < I.nni_node.size(); ++k) {
+                const size_t b0 = ops.size();
+                if (append_nni_program(&I, par, I.nni_node[k], I.nni_side[k], ops)) {
+                    I.nni_prog.push_back((int)prog_begin.size() - 1);
+                    prog_begin.push_back((int)ops.size());
+                    prog_len.push_back((int)(ops.size() - b0));
+                    I.nni_ops += (long)(ops.size() - b0);
+                } else {
+                    I.nni_prog.push_back(-1);
+                }
+            }
+            b->updates2 += (double)I.nni_ops * I.m * b->C;
+        }
+        b->var_offsets.push_back(b->var_offsets.back() + (int)I.nni_node.size());
+    }
+    b->n_var = b->var_offsets.back();
+    // device buffers
+    const size_t elt = b->prec == PHYLO_PREC_FP64 ? sizeof(double) : sizeof(float);
+    HIPCHK(b, hipMalloc(&b->d_tips, std::max<long>(tips_bytes, 1)));
+    HIPCHK(b, hipMalloc(&b->d_clv, std::max<size_t>((size_t)clv_elems * elt, 16)));
+    HIPCHK(b, hipMalloc(&b->d_scl, std::max<size_t>((size_t)scl_elems * sizeof(int), 16)));
+    HIPCHK(b, hipMalloc(&b->d_P, std::max<size_t>((size_t)P_elems * sizeof(double), 16)));
+    HIPCHK(b, hipMalloc(&b->d_roots, (size_t)roots_elems * sizeof(double)));
+    HIPCHK(b, hipMalloc(&b->d_ops, std::max<size_t>(ops.size(), 1) * sizeof(Op)));
+    HIPCHK(b, hipMalloc(&b->d_prog, prog_begin.size() * sizeof(int)));
+    HIPCHK(b, hipMalloc(&b->d_inst, b->inst.size() * sizeof(BatchInst)));
+    std::vector<BatchInst> hinst;
+    std::vector<uint8_t> codes;
+    for (auto& I : b->inst) {
+        BatchInst d{};
+        d.tips_off = I.tips_off;
+        d.clv_off = I.clv_off;
+        d.scl_off = I.scl_off;
+        d.P_off = I.P_off;
+        d.roots_off = I.roots_off;
+        d.m = I.m;
+        d.m_pad = I.m_pad;
+        hinst.push_back(d);
+        codes.assign((size_t)I.n_taxa * I.m_pad, 0x0F);
+        for (int t = 0; t < I.n_taxa; ++t)
+            for (int s = 0; s < I.m; ++s) codes[(size_t)t * I.m_pad + s] = (uint8_t)I.masks[(size_t)t * I.m + s];
+        HIPCHK(b, hipMemcpy(b->d_tips + I.tips_off, codes.data(), codes.size(), hipMemcpyHostToDevice));
+        HIPCHK(b, hipMemcpy(b->d_P + I.P_off, I.P.data(), I.P.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(b, hipMemcpy(b->d_roots + I.roots_off, I.roots, sizeof(I.roots), hipMemcpyHostToDevice));
+        std::vector<uint32_t>().swap(I.masks);
+        std::vector<double>().swap(I.P);
+    }
+    HIPCHK(b, hipMemcpy(b->d_inst, hinst.data(), hinst.size() * sizeof(BatchInst), hipMemcpyHostToDevice));
+    HIPCHK(b, hipMemcpy(b->d_ops, ops.data(), ops.size() * sizeof(Op), hipMemcpyHostToDevice));
+    HIPCHK(b, hipMemcpy(b->d_prog, prog_begin.data(), prog_begin.size() * sizeof(int), hipMemcpyHostToDevice));
+    const int n_out = (int)b->inst.size() + b->n_var;
+    HIPCHK(b, hipMalloc(&b->d_out, std::max(n_out, 1) * sizeof(double)));
+    HIPCHK(b, hipHostMalloc(reinterpret_cast<void**>(&b->h_out), std::max(n_out, 1) * sizeof(double)));
+    // persistent grid: as many workgroups as can be resident at once
+    hipDeviceProp_t prop;
+    HIPCHK(b, hipGetDeviceProperties(&prop, b->device));
+    int per_cu = 1;
+    const int threads = 64 * b->C;
+    const bool f64 = b->prec == PHYLO_PREC_FP64;
+    const void* kfn = nullptr;
+    switch (b->C) {
+    case 1: kfn = f64 ? (const void*)batch4_kernel<double, 1> : (const void*)batch4_kernel<float, 1>; break;
+    case 2: kfn = f64 ? (const void*)batch4_kernel<double, 2> : (const void*)batch4_kernel<float, 2>; break;
+    case 4: kfn = f64 ? (const void*)batch4_kernel<double, 4> : (const void*)batch4_kernel<float, 4>; break;
+    default: kfn = f64 ? (const void*)batch4_kernel<double, 8> : (const void*)batch4_kernel<float, 8>; break;
+    }
+    HIPCHK(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, threads, 0));
+    b->grid = std::max(1, per_cu) * prop.multiProcessorCount;
+    // autotune the number of 64-site blocks per work item
+    std::vector<int> cands = autotune ? std::vector<int>{1, 2, 4, 8} : std::vector<int>{2};
+    float best = 1e30f;
+    int best_bpi = cands.front();
+    for (int bpi : cands) {
+        build_work_lists(b, bpi, prog_len);
+        int rc = upload_work_lists(b);
+        if (rc) return rc;
+        b->blocks_per_item = bpi;
+        float ms = 0.f;
+        for (int rep = 0; rep < (autotune ? 2 : 0); ++rep) {
+            HIPCHK(b, hipEventRecord(b->ev0, b->stream));
+            rc = enqueue_batch_run(b);
+            if (rc) return rc;
+            HIPCHK(b, hipEventRecord(b->ev1, b->stream));
+            HIPCHK(b, hipStreamSynchronize(b->stream));
+            HIPCHK(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        }
+        if (autotune && ms < best) { best = ms; best_bpi = bpi; }
+    }
+    if (b->blocks_per_item != best_bpi) {
+        build_work_lists(b, best_bpi, prog_len);
+        int rc = upload_work_lists(b);
+        if (rc) return rc;
+        b->blocks_per_item = best_bpi;
+    }
+    // capture the whole run as a graph, replayed by phylo_batch_run
+    HIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_batch_run(b);
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(b->stream, &g);
+    if (rc) return rc;
+    HIPCHK(b, ce);
+    b->graph = g;
+    HIPCHK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->finalized = true;
+    return PHYLO_OK;
+}
+
+int phylo_batch_run(phylo_batch* b, double* totals, double* nni_lnl) {
+    if (!b) return PHYLO_ERR_ARG;
+    if (!b->finalized) return fail(b, PHYLO_ERR_STATE, "batch_run before batch_finalize");
+    HIPCHK(b, hipSetDevice(b->device));
+    HIPCHK(b, hipEventRecord(b->ev0, b->stream));
+    HIPCHK(b, hipGraphLaunch(b->exec, b->stream));
+    HIPCHK(b, hipEventRecord(b->ev1, b->stream));
+    HIPCHK(b, hipStreamSynchronize(b->stream));
+    float ms = 0.f;
+    HIPCHK(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    const int ni = (int)b->inst.size();
+    if (totals)
+        for (int i = 0; i < ni; ++i) totals[i] = b->h_out[i];
+    if (nni_lnl) {
+        int seg = 0;
+        for (int i = 0; i < ni; ++i) {
+            const auto& I = b->inst[i];
+            for (size_t k = 0; k < I.nni_prog.size(); ++k) {
+                double v = b->h_out[i];                  // no-op variants keep the base tree
+                if (I.nni_prog[k] >= 0) v = b->h_out[ni + seg++];
+                nni_lnl[b->var_offsets[i] + k] = v;
+            }
+        }
+    }
+    return PHYLO_OK;
+}
+
+int phylo_batch_nni_layout(phylo_batch* b, int32_t* offsets, int32_t* node, int32_t* side) {
+    if (!b) return PHYLO_ERR_ARG;
+    if (!b->finalized) return fail(b, PHYLO_ERR_STATE, "batch_nni_layout before batch_finalize");
+    for (size_t i = 0; i < b->var_offsets.size() && offsets; ++i) offsets[i] = b->var_offsets[i];
+    for (size_t i = 0; i < b->inst.size(); ++i)
+        for (size_t k = 0; k < b->inst[i].nni_node.size(); ++k) {
+            if (node) node[b->var_offsets[i] + k] = b->inst[i].nni_node[k];
+            if (side) side[b->var_offsets[i] + k] = b->inst[i].nni_side[k];
+        }
+    return PHYLO_OK;
+}
+
+int phylo_batch_stats(phylo_batch* b, double* updates_full, double* updates_nni, double* last_ms,
+                      int* sites_per_item, int* n_variants) {
+    if (!b) return PHYLO_ERR_ARG;
+    if (updates_full) *updates_full = b->updates1;
+    if (updates_nni) *updates_nni = b->updates2;
+    if (last_ms) *last_ms = b->last_ms;
+    if (sites_per_item) *sites_per_item = 64 * b->blocks_per_item;
+    if (n_variants) *n_variants = b->n_var;
+    return PHYLO_OK;
+}
+
+void phylo_batch_destroy(phylo_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    void* bufs[] = {b->d_tips, b->d_clv, b->d_scl, b->d_P, b->d_roots, b->d_ops, b->d_prog, b->d_inst,
+                    b->d_items1, b->d_items2, b->d_partial, b->d_seg1, b->d_seg2, b->d_out};
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
+    if (b->h_out) (void)hipHostFree(b->h_out);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
 }
 
 }  // extern "C"
