@@ -98,6 +98,115 @@ def cpu_baseline(wl, budget_s=20.0):
             "sample": f"oracle restatement of simpleML's cost model, {n} taxa x {m_sample} sites, 1 category"}
 
 
+def bench_protein(args, P, synth):
+    """20-state protein, 64-taxon Yule tree, 20,000 sites, 190 exchangeabilities
+    ~ Gamma(1,1), frequencies ~ Dirichlet(2x20), gamma shape 1.0 (4 categories),
+    3% gaps."""
+    wl = synth.gtr_gamma_workload(n_tips=64, m=20_000, S=20, C=4, alpha=1.0, gap_frac=0.03, seed=4,
+                                  rate_dist="gamma", freq_conc=2.0, mean_blen=0.05)
+    t = wl.tree
+    ctx = P.Context(0, P.PREC_FP32)
+    ctx.set_tips(wl.masks, 20)
+    ctx.set_topology(t.left, t.right, t.root)
+    ctx.set_edge_lengths(t.blen)
+    ctx.set_model(wl.Q, wl.pi, wl.pi, P.P_EIGEN)
+    ctx.set_categories(wl.cat_rates, wl.cat_weights)
+    for _ in range(args.warmup + 1):
+        ctx.loglik()
+    ctx.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        lnl = ctx.loglik()
+        kms.append(ctx.last_kernel_ms())
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    n, m, C, S = 64, 20_000, 4, 20
+    updates = (n - 1) * m * C
+    ks = float(np.mean(kms)) * 1e-3
+    per_site = C * S * 4 * (n - 2) + n + C * S * 4 * (n - 1) + 4 * (n - 1) + 4 * (n - 2)
+    bpu = per_site / ((n - 1) * C)
+    flops = 2 * 2 * S * S + S
+    return {"metric": "CLV updates/s (site*node*cat)", "value": updates * args.steps / el, "unit": "updates/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded Yule tree, 20-state simulation)",
+            "config": {"workload": "20-state protein + G4, 64 taxa Yule, 20000 sites, 3% gaps", "n_taxa": n,
+                       "n_sites": m, "n_states": S, "n_cat": C, "clv_storage": "f32", "lnL": lnl},
+            "roofline": {"bound": "hbm", "achieved": bpu * updates / ks / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": bpu * updates / ks / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                         "kernel": "prune_wide_kernel<float,4,20>", "kernel_ms": ks * 1e3,
+                         "alg_bytes_per_update": bpu,
+                         "fp64_tflops": flops * updates / ks / 1e12, "fp64_peak_tflops": 78.6}}
+
+
+def bench_nni(args, P, synth):
+    """Full NNI neighbourhood (2*(n-3) = 250 variants) of the 128-taxon x 100k GTR+G4 tree."""
+    wl = synth.gtr_gamma_workload(n_tips=128, m=100_000, mean_blen=0.05, alpha=0.5, C=4, gap_frac=0.05, seed=2)
+    t = wl.tree
+    ctx = P.Context(0, P.PREC_FP32)
+    ctx.set_tips(wl.masks, 4)
+    ctx.set_topology(t.left, t.right, t.root)
+    ctx.set_edge_lengths(t.blen)
+    ctx.set_model(wl.Q, wl.pi, wl.pi, P.P_EIGEN)
+    ctx.set_categories(wl.cat_rates, wl.cat_weights)
+    ctx.loglik()
+    for _ in range(args.warmup):
+        ctx.nni_score(1)
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        base, var = ctx.nni_score(1)
+        kms.append(ctx.last_kernel_ms())
+    el = time.perf_counter() - t0
+    ops = ctx.nni_ops()
+    updates = ops * 100_000 * 4
+    ks = float(np.mean(kms)) * 1e-3
+    full_equiv = len(var) * 127 * 100_000 * 4
+    return {"metric": "CLV updates/s (site*node*cat)", "value": updates * args.steps / el, "unit": "updates/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic", "config": {"workload": "full NNI neighbourhood of GTR+G4 128 taxa x 100000 sites",
+                                            "variants": len(var), "recompute_ops": ops,
+                                            "full_reevaluation_updates": full_equiv,
+                                            "variants_per_s": len(var) * args.steps / el},
+            "kernel_ms": ks * 1e3}
+
+
+def bench_batch(args, P, synth, n_inst=512):
+    """512 independent GTR+G4 alignments, taxa ~ U{8..256}, sites log-uniform
+    [500, 50000], per-instance Dirichlet parameters, each with its full NNI
+    neighbourhood, scored by the persistent batch launches replayed from one
+    captured HIP graph."""
+    rng = np.random.default_rng(3)
+    b = P.Batch(4, P.PREC_FP32)
+    t0 = time.time()
+    for i in range(n_inst):
+        n = int(rng.integers(8, 257))
+        m = int(np.exp(rng.uniform(np.log(500), np.log(50_000))))
+        wl = synth.gtr_gamma_workload(n_tips=n, m=m, C=4, alpha=float(rng.uniform(0.3, 2.0)), gap_frac=0.05,
+                                      seed=10_000 + i, mean_blen=0.05)
+        t = wl.tree
+        b.add(wl.masks, t.left, t.right, t.root, t.blen, wl.Q, wl.pi, wl.cat_rates, wl.cat_weights, nni_mode=1)
+        if i % 64 == 63:
+            print(f"[bench] batch: {i + 1} instances added ({time.time() - t0:.0f}s)", file=sys.stderr, flush=True)
+    b.finalize(autotune=True)
+    for _ in range(args.warmup):
+        b.run()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        tot, nni = b.run()
+    el = time.perf_counter() - t1
+    u1, u2, ms, spi, nv = b.stats()
+    return {"metric": "CLV updates/s (site*node*cat)", "value": (u1 + u2) * args.steps / el, "unit": "updates/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic", "config": {"workload": f"{n_inst} GTR+G4 alignments + full NNI neighbourhoods",
+                                            "updates_full": u1, "updates_nni": u2, "variants": nv,
+                                            "sites_per_item": spi, "device_ms": ms,
+                                            "finite": bool(np.all(np.isfinite(tot)) and np.all(np.isfinite(nni)))}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,7 +216,22 @@ def main():
     ap.add_argument("--taxa", type=int, default=128)
     ap.add_argument("--fp64", action="store_true", help="store partials in FP64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="dna", choices=["dna", "protein", "nni", "batch"])
+    ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="run only a plain device-to-device copy of MIB MiB (torch), the calibration kernel "
+                         "for the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_mib": args.copy_probe, "bytes_read": a.numel() * 4,
+                          "bytes_written": b.numel() * 4}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               f"(sites shard across GPUs with one 8-byte all-reduce per evaluation; not built here)",
@@ -116,6 +240,12 @@ def main():
 
     import phylommand_amd as P
     from phylommand_amd import synth
+
+    if args.workload != "dna":
+        fn = {"protein": bench_protein, "nni": bench_nni}.get(args.workload)
+        out = fn(args, P, synth) if fn else bench_batch(args, P, synth, args.instances)
+        print(json.dumps(out))
+        return
 
     t0 = time.time()
     wl = synth.gtr_gamma_workload(n_tips=args.taxa, m=args.sites, mean_blen=0.05, alpha=0.5, C=4,
