@@ -36,8 +36,31 @@ def build_hip(verbose: bool = True) -> str:
     return LIB
 
 
+HOST = os.path.join(HERE, "host")
+HOST_SRCS = ["sub_model.cpp", "rate_model.cpp", "tree.cpp", "io.cpp", "simpleML.cpp"]
+HOST_LIB = os.path.join(HERE, "libphylommand_host.so")
+HOST_DRIVER = os.path.join(HOST, "host_driver")
+CXX_FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-Wall", "-Wno-unknown-pragmas", "-fPIC"]
+
+
+def build_host():
+    """The simpleML / sub_model / rate_model host classes over the C ABI, and
+    their command-line driver."""
+    srcs = [os.path.join(HOST, f) for f in HOST_SRCS]
+    inc = ["-I" + os.path.join(ROOT, "include"), "-I" + HOST]
+    link = ["-L" + HERE, "-lphylo_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + HERE]
+    tmp = HOST_LIB + ".tmp"
+    run(["g++", *CXX_FLAGS, "-shared", *inc, "-o", tmp, *srcs, *link])
+    os.replace(tmp, HOST_LIB)
+    tmp = HOST_DRIVER + ".tmp"
+    run(["g++", *CXX_FLAGS, *inc, "-o", tmp, os.path.join(HOST, "host_driver.cpp"), "-L" + HERE,
+         "-lphylommand_host", "-lphylo_hip", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + HERE])
+    os.replace(tmp, HOST_DRIVER)
+
+
 def build_all():
     build_hip()
+    build_host()
 
 
 if __name__ == "__main__":

@@ -1,0 +1,217 @@
+// host_driver.cpp -- command-line driver of the simpleML/sub_model/rate_model
+// host classes, used by the tests to run the same inputs the reference's
+// treeator path takes.  Modes (numbers printed with %.17g):
+//   persite  TREE FASTA ALPHA TR SPEC PARAMS FREQS [fp64|fp32] [auto|refcompat|eigen|pade]
+//            per-site lnL of one multi-site evaluation, then "total X"
+//   ratelik  TREE FASTA ALPHA TR SPEC PARAMS FREQS TIMEARG CLADEARG
+//   nodes    TREE FASTA ALPHA TR SPEC PARAMS FREQS NSITES   (labels via draw_normalized...)
+//   nni      TREE          the tree and its treebender --nni all variants
+//   nniscore TREE FASTA ALPHA TR SPEC PARAMS FREQS MODE     NNI neighbourhood lnL
+//   parse    TREE          print_newick of every tree in the file
+//   tmbl     TIMEARG B D [B D ...]
+//   pmatrix  S TR SPEC PARAMS FREQS T [T ...]
+//   simulate TREE ALPHA TR SPEC PARAMS FREQS NCHAR SEED    fasta on stdout
+// ALPHA is dna | protein | binary | an alphabet file; lists are comma
+// separated, "-" for none.
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+
+#include "io.h"
+#include "rate_model.h"
+#include "simpleML.h"
+#include "sub_model.h"
+
+using namespace phylo_amd;
+
+static std::vector<double> dlist(const std::string& s) {
+    std::vector<double> v;
+    if (s == "-") return v;
+    std::stringstream ss(s);
+    std::string tok;
+    while (std::getline(ss, tok, ',')) v.push_back(atof(tok.c_str()));
+    return v;
+}
+
+static void load_alphabet(const std::string& name, alphabet_map& a) {
+    a.clear();
+    if (name == "dna") alphabet::set_alphabet_dna(a);
+    else if (name == "protein") alphabet::set_alphabet_amino_acid(a);
+    else if (name == "binary") alphabet::set_alphabet_binary(a);
+    else {
+        std::ifstream f(name);
+        alphabet::parse_alphabet_file(f, a);
+    }
+}
+
+struct Data {
+    std::vector<character_vector> rows;
+    alphabet_map alpha;
+    unsigned int n_states = 0;
+};
+
+// treeator.cpp:395-466, :555-558
+static void load_data(const char* fasta, const std::string& alpha, Data& d) {
+    load_alphabet(alpha, d.alpha);
+    std::ifstream f(fasta);
+    parse_fasta(f, d.alpha, d.rows);
+    const state_set all = alphabet::all_states(d.alpha);
+    for (auto& r : d.rows) r.set_empty_to(all);
+    d.n_states = n_states_of(d.rows);
+}
+
+// treeator.cpp:573-636
+static bool setup_model(sub_model& model, const std::string& spec, const std::string& params,
+                        const std::string& freqs) {
+    const std::vector<double> sp = dlist(spec), pa = dlist(params), fr = dlist(freqs);
+    for (unsigned int i = 0; i < sp.size(); ++i)
+        if (!model.set_rate_spec(i, (unsigned int)sp[i])) return false;
+    if (pa.size() != model.get_n_rates()) return false;
+    for (unsigned int i = 0; i < pa.size(); ++i)
+        if (!model.set_rate(i, pa[i])) return false;
+    if (model.is_tr()) {
+        if (fr.empty()) model.set_freq_same();
+        else
+            for (unsigned int i = 0; i < fr.size(); ++i) model.set_freq(i, fr[i]);
+    }
+    return true;
+}
+
+static std::string slurp(const char* path) {
+    std::ifstream f(path);
+    std::stringstream ss;
+    ss << f.rdbuf();
+    return ss.str();
+}
+
+static void fill_chars(simpleML& t, Data& d, sub_model& model, unsigned int n) {
+    t.init();
+    for (auto& r : d.rows)
+        for (unsigned int s = 0; s < n && s < r.n_char(); ++s) t.set_char(s, r.get_taxon(), r.get_character(s), model);
+}
+
+int main(int argc, char** argv) {
+    if (argc < 3) { std::cerr << "usage: see header" << std::endl; return 2; }
+    const std::string mode = argv[1];
+    if (mode == "persite" || mode == "ratelik" || mode == "nodes" || mode == "nniscore") {
+        if (argc < 9) return 2;
+        Data d;
+        load_data(argv[3], argv[4], d);
+        sub_model model(d.n_states, atoi(argv[5]) != 0);
+        if (!setup_model(model, argv[6], argv[7], argv[8])) { std::cerr << "model setup failed" << std::endl; return 3; }
+        std::stringstream ts(slurp(argv[2]));
+        simpleML t;
+        t.tree_file_parser(ts);
+        const unsigned int n_char = d.rows.empty() ? 0 : d.rows[0].n_char();
+        if (mode == "persite") {
+            if (argc > 9) t.set_precision_fp64(std::string(argv[9]) == "fp64");
+            if (argc > 10) {
+                const std::string b = argv[10];
+                t.set_p_builder(b == "refcompat" ? simpleML::P_REFCOMPAT : b == "eigen" ? simpleML::P_EIGEN
+                                : b == "pade" ? simpleML::P_PADE : simpleML::P_AUTO);
+            }
+            fill_chars(t, d, model, n_char);
+            const double total = t.calculate_log_likelihood(model);
+            for (double x : t.site_log_likelihoods()) printf("%.17g\n", x);
+            printf("total %.17g\n", total);
+        } else if (mode == "ratelik") {
+            if (argc < 11) return 2;
+            rate_model rate_mod;
+            std::vector<std::string> sets;
+            const std::string targ = argv[9], carg = argv[10];
+            if (targ != "-" && !rate_mod.pars_rates_and_times_arg(targ.c_str())) return 4;
+            if (carg != "-" && !rate_mod.pars_clade_rates_arg(carg.c_str(), sets)) return 4;
+            if (!sets.empty()) t.add_taxon_sets(sets);
+            t.set_precision_fp64(true);
+            t.set_p_builder(simpleML::P_REFCOMPAT);
+            fill_chars(t, d, model, n_char);
+            const double l = carg != "-" ? t.calculate_likelihood_rate_change_at_nodes(rate_mod, model)
+                                         : t.calculate_likelihood_rate_change_in_time(rate_mod, model);
+            printf("total %.17g\n", l);
+        } else if (mode == "nodes") {
+            if (argc < 10) return 2;
+            const unsigned int ns = (unsigned int)atoi(argv[9]);
+            t.set_precision_fp64(true);
+            t.set_p_builder(simpleML::P_REFCOMPAT);
+            fill_chars(t, d, model, ns);
+            printf("total %.17g\n", t.calculate_log_likelihood(model));
+            fflush(stdout);
+            t.draw_normalized_likelihood_on_nodes();
+            t.print_newick();
+        } else {
+            if (argc < 10) return 2;
+            fill_chars(t, d, model, n_char);
+            std::vector<int> node, side;
+            const std::vector<double> l = t.nni_neighbourhood(model, atoi(argv[9]), &node, &side);
+            for (size_t k = 0; k < l.size(); ++k) printf("%d %d %.17g\n", node[k], side[k], l[k]);
+        }
+        return 0;
+    }
+    if (mode == "parse" || mode == "nni") {
+        std::stringstream ts(slurp(argv[2]));
+        while (true) {
+            tree t;
+            t.tree_file_parser(ts);
+            if (t.empty()) break;
+            if (mode == "parse") { t.print_newick(); continue; }
+            std::vector<tree> out{t};
+            const unsigned int n = t.n_tips();
+            if (n >= 4)
+                for (unsigned int i = 2; i <= n - 3; ++i) {
+                    tree a, b;
+                    t.nni(i, a, b);
+                    out.push_back(a);
+                    out.push_back(b);
+                }
+            for (auto& x : out) x.print_newick();
+            break;
+        }
+        return 0;
+    }
+    if (mode == "tmbl") {
+        rate_model rate_mod;
+        const std::string targ = argv[2];
+        if (targ != "-" && !rate_mod.pars_rates_and_times_arg(targ.c_str())) return 4;
+        for (int a = 3; a + 1 < argc; a += 2)
+            printf("%.17g\n", rate_mod.get_time_mod_branch_length(atof(argv[a]), (float)atof(argv[a + 1])));
+        return 0;
+    }
+    if (mode == "pmatrix") {
+        if (argc < 8) return 2;
+        const unsigned int S = (unsigned int)atoi(argv[2]);
+        sub_model model(S, atoi(argv[3]) != 0);
+        if (!setup_model(model, argv[4], argv[5], argv[6])) return 3;
+        for (int a = 7; a < argc; ++a) {
+            model.set_P_matrix(atof(argv[a]));
+            for (unsigned int i = 0; i < S; ++i)
+                for (unsigned int j = 0; j < S; ++j) printf("%.17g%c", model.get_P_value(i, j), j + 1 < S ? ' ' : '\n');
+        }
+        return 0;
+    }
+    if (mode == "simulate") {
+        if (argc < 10) return 2;
+        Data d;
+        load_alphabet(argv[3], d.alpha);
+        unsigned int S = 0;
+        for (const auto& kv : d.alpha)
+            for (unsigned int j = S; j < SIZE; ++j)
+                if (kv.second.test(j)) S = j + 1;
+        sub_model model(S, atoi(argv[4]) != 0);
+        if (!setup_model(model, argv[5], argv[6], argv[7])) return 3;
+        std::stringstream ts(slurp(argv[2]));
+        simpleML t;
+        t.tree_file_parser(ts);
+        t.set_simulation_seed((uint64_t)atoll(argv[9]));
+        std::vector<double> freq = model.frequencies();
+        for (auto& r : t.simulate_chars(freq, (unsigned int)atoi(argv[8]), model)) {
+            std::cout << '>' << r.get_taxon() << '\n';
+            for (unsigned int j = 0; j < r.n_char(); ++j) std::cout << alphabet::translate_bitset(r.character(j), d.alpha);
+            std::cout << '\n';
+        }
+        return 0;
+    }
+    std::cerr << "unknown mode" << std::endl;
+    return 2;
+}

@@ -1,0 +1,111 @@
+"""The simpleML / sub_model / rate_model host classes on the GPU (through
+host_driver) against the golden vectors the compiled reference wrote for the
+same trees, alignments and models."""
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from golden_cases import GOLDEN, PERSITE_CASES, read_expected
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "phylommand_amd", "host", "host_driver")
+
+
+def drive(*args):
+    out = subprocess.run([DRIVER, *map(str, args)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    return out.stdout
+
+
+def case_args(name):
+    d = os.path.join(GOLDEN, name)
+    meta = json.load(open(os.path.join(d, "case.json")))
+    alpha = meta["alphabet"]
+    if alpha not in ("dna", "protein", "binary"):
+        alpha = os.path.join(d, alpha)
+    return [os.path.join(d, "tree.nwk"), os.path.join(d, "aln.fst"), alpha, meta["tr"], meta["spec"],
+            meta["params"], meta["freqs"]]
+
+
+def parse_out(text):
+    site, total = [], None
+    for line in text.strip().split("\n"):
+        if line.startswith("total"):
+            total = float(line.split()[1])
+        else:
+            site.append(float(line))
+    return np.array(site), total
+
+
+@pytest.mark.parametrize("name", PERSITE_CASES)
+def test_host_persite_refcompat_fp64(name):
+    print(f"host persite {name}", flush=True)
+    site, total = parse_out(drive("persite", *case_args(name), "fp64", "refcompat"))
+    want, want_total = read_expected(os.path.join(GOLDEN, name, "expected.txt"))
+    np.testing.assert_allclose(site, want, rtol=1e-11, atol=1e-13)
+    assert abs(total - want_total) <= 1e-11 * abs(want_total)
+
+
+@pytest.mark.parametrize("name", PERSITE_CASES)
+def test_host_persite_default_engine(name):
+    """Default engine settings: FP32 partials, exact P(t)."""
+    site, total = parse_out(drive("persite", *case_args(name)))
+    want, want_total = read_expected(os.path.join(GOLDEN, name, "expected.txt"))
+    assert np.max(np.abs(site - want) / np.abs(want)) <= 1e-5
+    assert abs(total - want_total) <= 1e-3
+
+
+def test_host_example_files():
+    d = os.path.join(GOLDEN, "example_files")
+    meta = json.load(open(os.path.join(d, "case.json")))
+    for name, (spec, par, fr) in meta["models"].items():
+        site, total = parse_out(drive("persite", os.path.join(d, "tree_file.tree"),
+                                      os.path.join(d, "alignment_file.fst"), "dna", 1, spec, par, fr, "fp64",
+                                      "refcompat"))
+        want, want_total = read_expected(os.path.join(d, f"expected_{name}.txt"))
+        np.testing.assert_allclose(site, want, rtol=1e-11)
+
+
+def test_host_rate_models():
+    d = os.path.join(GOLDEN, "ratemodel_8x50")
+    for case in json.load(open(os.path.join(d, "expected.json"))):
+        out = drive("ratelik", os.path.join(d, "tree.nwk"), os.path.join(d, "aln.fst"), "dna", 1, "0,0,0,0,0,0",
+                    "1.3333333333333333", "0.25,0.25,0.25,0.25", case["time_arg"], case["clade_arg"])
+        total = float(out.split()[1])
+        assert abs(total - case["total"]) <= 1e-11 * abs(case["total"]), case
+
+
+def test_host_node_labels():
+    d = os.path.join(GOLDEN, "nodes_6x3")
+    out = drive("nodes", os.path.join(d, "tree.nwk"), os.path.join(d, "aln.fst"), "dna", 1, "0,1,0,0,1,0",
+                "1.0,0.25", "0.1,0.2,0.3,0.4", 3)
+    want = open(os.path.join(d, "expected.txt")).read()
+    got_tree = out.strip().split("\n")[1]
+    want_tree = want.strip().split("\n")[1]
+    nums = lambda s: [float(x) for x in re.findall(r"[-+]?\d*\.?\d+(?:[eE][-+]?\d+)?", s)]
+    # same structure and labels; numbers agree to the 6 printed digits
+    assert re.sub(r"[-+]?\d*\.?\d+(?:[eE][-+]?\d+)?", "#", got_tree) == \
+        re.sub(r"[-+]?\d*\.?\d+(?:[eE][-+]?\d+)?", "#", want_tree)
+    np.testing.assert_allclose(nums(got_tree), nums(want_tree), rtol=2e-5, atol=1e-9)
+
+
+def test_host_nni_scores_match_reevaluation(tmp_path):
+    """simpleML::nni_neighbourhood (mode 0, treebender order) against scoring
+    each tree of host_driver's own treebender-style NNI list from scratch."""
+    d = os.path.join(GOLDEN, "dna3_24x300")
+    args = case_args("dna3_24x300")
+    out = drive("nniscore", *args, 0)
+    scores = [float(line.split()[2]) for line in out.strip().split("\n")]
+    trees = drive("nni", args[0]).strip().split("\n")[1:]
+    assert len(trees) == len(scores)
+    for k in (0, 1, len(trees) // 2, len(trees) - 1):
+        p = tmp_path / f"v{k}.nwk"
+        p.write_text(trees[k] + "\n")
+        _, tot = parse_out(drive("persite", p, *args[1:]))
+        # the variant tree was printed with 6-decimal lengths
+        assert abs(tot - scores[k]) <= 1e-3 * max(1.0, abs(tot)) + 0.05, (k, tot, scores[k])

@@ -1,0 +1,67 @@
+"""The C++ host classes (phylommand_amd/host: tree, sub_model, rate_model,
+io) against the golden outputs of the compiled reference.  CPU-only modes of
+host_driver; the GPU-backed modes are in test_gpu_host.py."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from golden_cases import GOLDEN
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "phylommand_amd", "host", "host_driver")
+
+
+def drive(*args):
+    if not os.path.exists(DRIVER):
+        pytest.skip("host_driver not built (python -m phylommand_amd.build)")
+    out = subprocess.run([DRIVER, *map(str, args)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    return out.stdout
+
+
+def test_newick_parse_print_matches_reference(tmp_path):
+    d = os.path.join(GOLDEN, "parse")
+    inputs = open(os.path.join(d, "input.nwk")).read().strip().split("\n")
+    want = open(os.path.join(d, "expected.txt")).read().strip().split("\n")
+    for s, w in zip(inputs, want):
+        p = tmp_path / "t.nwk"
+        p.write_text(s + "\n")
+        assert drive("parse", p).strip() == w, s
+
+
+@pytest.mark.parametrize("n", [5, 8, 16])
+def test_nni_lists_match_treebender(n):
+    d = os.path.join(GOLDEN, "nni")
+    got = drive("nni", os.path.join(d, f"tree{n}.nwk")).strip().split("\n")
+    want = open(os.path.join(d, f"expected{n}.txt")).read().strip().split("\n")
+    assert got == want
+
+
+def test_time_mod_length_bit_exact():
+    for case in json.load(open(os.path.join(GOLDEN, "time_mod_length.json"))):
+        flat = [x for p in case["pairs"] for x in p]
+        got = [float(x) for x in drive("tmbl", case["arg"], *flat).split()]
+        assert got == case["out"], case["arg"]
+
+
+def test_pmatrix_bit_exact():
+    for key, c in json.load(open(os.path.join(GOLDEN, "pmatrix_refcompat.json"))).items():
+        out = drive("pmatrix", c["S"], c["tr"], c["spec"], c["params"], c["freqs"], *c["t"])
+        vals = np.array([[float(x) for x in line.split()] for line in out.strip().split("\n")])
+        want = np.array(c["P"]).reshape(-1, c["S"])
+        np.testing.assert_array_equal(vals, want, err_msg=key)
+
+
+def test_simulation_frequencies(tmp_path):
+    """simulate_chars (parity unpinned: the reference draws from time-seeded
+    rand()): long simulations reproduce the root frequencies at the tips."""
+    p = tmp_path / "t.nwk"
+    p.write_text("((a:0.3,b:0.2):0.1,(c:0.5,d:0.05):0.2);\n")
+    out = drive("simulate", p, "dna", 1, "0,1,0,0,1,0", "1.0,0.3", "0.1,0.2,0.3,0.4", 20000, 7)
+    rows = out.strip().split("\n")[1::2]
+    assert len(rows) == 4 and all(len(r) == 20000 for r in rows)
+    freq = np.array([sum(r.count(ch) for r in rows) for ch in "AGCT"]) / 80000
+    np.testing.assert_allclose(freq, [0.1, 0.2, 0.3, 0.4], atol=0.02)
