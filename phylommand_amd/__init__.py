@@ -27,6 +27,7 @@ _lib = None
 # (name, restype, argtypes) of every entry point in include/phylo_hip.h
 _c_int, _c_dbl_p, _c_int_p, _c_void_p = ctypes.c_int, ctypes.POINTER(ctypes.c_double), \
     ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p
+_c_flt_p = ctypes.POINTER(ctypes.c_float)
 SIGNATURES = [
     ("phylo_version", _c_int, []),
     ("phylo_create", _c_int, [_c_int, ctypes.POINTER(_c_void_p)]),
@@ -58,7 +59,34 @@ SIGNATURES = [
                                    ctypes.POINTER(ctypes.c_int)]),
     ("phylo_batch_destroy", None, [_c_void_p]),
     ("phylo_last_kernel_ms", _c_int, [_c_void_p, _c_dbl_p]),
+    # include/phylo_learn.h
+    ("phylo_sim_features", _c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int_p, _c_int_p, _c_int_p, _c_dbl_p,
+                                    _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.c_double, ctypes.c_uint64, _c_flt_p,
+                                    ctypes.POINTER(ctypes.c_uint32)]),
+    ("phylo_mlp_create", _c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, ctypes.c_uint64,
+                                  ctypes.POINTER(_c_void_p)]),
+    ("phylo_mlp_destroy", None, [_c_void_p]),
+    ("phylo_mlp_last_error", ctypes.c_char_p, [_c_void_p]),
+    ("phylo_mlp_n_params", ctypes.c_long, [_c_void_p]),
+    ("phylo_mlp_params", _c_int, [_c_void_p, _c_flt_p, _c_flt_p]),
+    ("phylo_mlp_forward", _c_int, [_c_void_p, _c_int, _c_flt_p, _c_flt_p]),
+    ("phylo_mlp_grad", _c_int, [_c_void_p, _c_int, _c_flt_p, _c_flt_p, _c_flt_p, _c_flt_p]),
+    ("phylo_mlp_set_data", _c_int, [_c_void_p, _c_int, _c_flt_p, _c_flt_p]),
+    ("phylo_mlp_train", _c_int, [_c_void_p, _c_int, _c_int, ctypes.c_float, ctypes.c_uint64, _c_flt_p]),
 ]
+
+# (name, restype, argtypes) of every entry point in include/phylo_host.h,
+# exported by libphylommand_host.so
+HOST_LIB_PATH = os.path.join(HERE, "libphylommand_host.so")
+HOST_SIGNATURES = [
+    ("phylo_gamma_rates", _c_int, [ctypes.c_double, _c_int, _c_dbl_p]),
+    ("phylo_gtr_gamma_lnl", ctypes.c_double, [_c_void_p, _c_int, _c_dbl_p, _c_int, _c_dbl_p]),
+    ("phylo_nm_defaults", _c_int, [ctypes.c_void_p]),
+    ("phylo_fit_gtr_gamma", _c_int, [_c_void_p, _c_int, _c_dbl_p, _c_int, _c_dbl_p, ctypes.c_void_p, _c_dbl_p,
+                                     _c_dbl_p, ctypes.POINTER(ctypes.c_int), _c_dbl_p]),
+]
+# which library exports the entry points of each header
+HEADER_LIBRARY = {"phylo_hip.h": LIB_PATH, "phylo_learn.h": LIB_PATH, "phylo_host.h": HOST_LIB_PATH}
 
 
 class PhyloError(RuntimeError):
@@ -83,6 +111,52 @@ def load_library(path: str = LIB_PATH):
     if path == LIB_PATH:
         _lib = lib
     return lib
+
+
+_host_lib = None
+
+
+class NMOptions(ctypes.Structure):
+    """phylo_nm_options (include/phylo_host.h)."""
+    _fields_ = [("max_evals", ctypes.c_int), ("ftol_abs", ctypes.c_double), ("xtol_rel", ctypes.c_double),
+                ("initial_step", ctypes.c_double), ("adaptive", ctypes.c_int), ("max_restarts", ctypes.c_int)]
+
+
+def nm_options(**kw) -> NMOptions:
+    """The library's Nelder-Mead defaults with the given fields replaced."""
+    o = NMOptions()
+    load_host_library().phylo_nm_defaults(ctypes.byref(o))
+    for k, v in kw.items():
+        if v is not None:
+            setattr(o, k, v)
+    return o
+
+
+def load_host_library():
+    """Loads libphylommand_host.so (the C++ host layer; it links
+    libphylo_hip.so, so contexts created through either are shared)."""
+    global _host_lib
+    if _host_lib is not None:
+        return _host_lib
+    load_library()
+    if not os.path.exists(HOST_LIB_PATH):
+        raise FileNotFoundError(f"{HOST_LIB_PATH} is missing: build it with `python -m phylommand_amd.build`")
+    lib = ctypes.CDLL(HOST_LIB_PATH)
+    for name, res, args in HOST_SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _host_lib = lib
+    return lib
+
+
+def gamma_rates(alpha: float, n_cat: int) -> np.ndarray:
+    """Mean rates of n_cat equal-probability discrete-gamma categories."""
+    r = np.zeros(n_cat)
+    rc = load_host_library().phylo_gamma_rates(float(alpha), int(n_cat), _dp(r))
+    if rc != PHYLO_OK:
+        raise PhyloError(rc, "phylo_gamma_rates: invalid arguments")
+    return r
 
 
 def _dp(a):
@@ -215,6 +289,25 @@ class Context:
         self._check(self.lib.phylo_last_kernel_ms(self.h, _dp(ms)))
         return float(ms[0])
 
+    def gtr_gamma_lnl(self, rel_len, n_cat: int, x) -> float:
+        """lnL of the GTR+Gamma point x (see include/phylo_host.h)."""
+        rel = np.ascontiguousarray(rel_len, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        return float(load_host_library().phylo_gtr_gamma_lnl(self.h, len(rel), _dp(rel), int(n_cat), _dp(x)))
+
+    def fit_gtr_gamma(self, rel_len, n_cat: int, x0, opts: "NMOptions | None" = None):
+        """Nelder-Mead GTR+Gamma fit from x0 (opts: nm_options(...), None for
+        the defaults); returns (x, lnl, evals, seconds)."""
+        rel = np.ascontiguousarray(rel_len, dtype=np.float64)
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        x = np.zeros(10)
+        lnl, secs = np.zeros(1), np.zeros(1)
+        ev = ctypes.c_int(0)
+        self._check(load_host_library().phylo_fit_gtr_gamma(self.h, len(rel), _dp(rel), int(n_cat), _dp(x0),
+                                                            None if opts is None else ctypes.byref(opts), _dp(x),
+                                                            _dp(lnl), ctypes.byref(ev), _dp(secs)))
+        return x, float(lnl[0]), ev.value, float(secs[0])
+
 
 class Batch:
     """Many independent DNA problems (and their NNI neighbourhoods) scored by
@@ -292,3 +385,92 @@ class Batch:
 
     def __exit__(self, *a):
         self.close()
+
+
+def _fp(a):
+    return a.ctypes.data_as(_c_flt_p)
+
+
+def sim_features(n_taxa, n_sites, left, right, roots, blen, eig, pi, cat_rates, gap_frac=0.0, seed=1,
+                 want_masks=False, device=0):
+    """Simulates len(roots) alignments on the GPU and returns their features
+    [n_align, 256] (and the tip masks [n_align, n_taxa, n_sites])."""
+    lib = load_library()
+    left = np.ascontiguousarray(left, dtype=np.int32)
+    right = np.ascontiguousarray(right, dtype=np.int32)
+    roots = np.ascontiguousarray(roots, dtype=np.int32)
+    blen = np.ascontiguousarray(blen, dtype=np.float64)
+    eig = np.ascontiguousarray(eig, dtype=np.float64)
+    pi = np.ascontiguousarray(pi, dtype=np.float64)
+    cat_rates = np.ascontiguousarray(cat_rates, dtype=np.float64)
+    n_align = len(roots)
+    n_cat = cat_rates.shape[1]
+    feat = np.zeros((n_align, 256), np.float32)
+    masks = np.zeros((n_align, n_taxa, n_sites), np.uint32) if want_masks else None
+    rc = lib.phylo_sim_features(device, n_align, n_taxa, n_sites, n_cat, _ip(left), _ip(right), _ip(roots), _dp(blen),
+                                _dp(eig), _dp(pi), _dp(cat_rates), float(gap_frac), int(seed), _fp(feat),
+                                masks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)) if want_masks else None)
+    if rc != PHYLO_OK:
+        raise PhyloError(rc, "phylo_sim_features failed")
+    return (feat, masks) if want_masks else feat
+
+
+class MLP:
+    """n_in -> n_hidden -> n_hidden -> n_out ReLU MLP trained on the GPU."""
+
+    def __init__(self, n_in=256, n_hidden=512, n_out=12, max_batch=256, seed=1, device=0):
+        self.lib = load_library()
+        h = _c_void_p()
+        rc = self.lib.phylo_mlp_create(device, n_in, n_hidden, n_out, max_batch, seed, ctypes.byref(h))
+        if rc != PHYLO_OK:
+            raise PhyloError(rc, "phylo_mlp_create failed")
+        self.h = h
+        self.n_in, self.n_hidden, self.n_out, self.max_batch = n_in, n_hidden, n_out, max_batch
+        self.n_params = self.lib.phylo_mlp_n_params(self.h)
+
+    def _check(self, rc):
+        if rc != PHYLO_OK:
+            raise PhyloError(rc, self.lib.phylo_mlp_last_error(self.h).decode())
+
+    def params(self):
+        w = np.zeros(self.n_params, np.float32)
+        self._check(self.lib.phylo_mlp_params(self.h, _fp(w), None))
+        return w
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == self.n_params
+        self._check(self.lib.phylo_mlp_params(self.h, None, _fp(w)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], self.n_out), np.float32)
+        self._check(self.lib.phylo_mlp_forward(self.h, x.shape[0], _fp(x), _fp(y)))
+        return y
+
+    def grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        self._check(self.lib.phylo_mlp_grad(self.h, x.shape[0], _fp(x), _fp(t), _fp(loss), _fp(g)))
+        return float(loss[0]), g
+
+    def train(self, x, t, epochs=20, batch=256, lr=1e-3, seed=1):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        self._check(self.lib.phylo_mlp_set_data(self.h, x.shape[0], _fp(x), _fp(t)))
+        losses = np.zeros(epochs, np.float32)
+        self._check(self.lib.phylo_mlp_train(self.h, epochs, batch, float(lr), int(seed), _fp(losses)))
+        return losses
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.phylo_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

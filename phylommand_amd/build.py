@@ -29,7 +29,7 @@ def run(cmd):
 
 
 def build_hip(verbose: bool = True) -> str:
-    srcs = [os.path.join(CSRC, "phylo_hip.hip")]
+    srcs = [os.path.join(CSRC, "phylo_hip.hip"), os.path.join(CSRC, "learn.hip")]
     tmp = LIB + ".tmp"
     run([HIPCC, *HIP_FLAGS, "-I" + os.path.join(ROOT, "include"), "-o", tmp, *srcs])
     os.replace(tmp, LIB)
@@ -37,7 +37,7 @@ def build_hip(verbose: bool = True) -> str:
 
 
 HOST = os.path.join(HERE, "host")
-HOST_SRCS = ["sub_model.cpp", "rate_model.cpp", "tree.cpp", "io.cpp", "simpleML.cpp"]
+HOST_SRCS = ["sub_model.cpp", "rate_model.cpp", "tree.cpp", "io.cpp", "simpleML.cpp", "optimize.cpp"]
 HOST_LIB = os.path.join(HERE, "libphylommand_host.so")
 HOST_DRIVER = os.path.join(HOST, "host_driver")
 CXX_FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-Wall", "-Wno-unknown-pragmas", "-fPIC"]

@@ -604,6 +604,15 @@ struct phylo_ctx {
     double* d_eig = nullptr; size_t eig_bytes = 0;
     double* d_eff = nullptr; size_t eff_bytes = 0;
     double* h_total = nullptr; size_t h_total_bytes = 0;  // pinned
+    // Pinned staging for the per-evaluation uploads (eigen system + edge
+    // lengths, root / category weights): copied without a stream
+    // synchronisation; an event guards reuse of the host buffer.
+    struct Stage {
+        double* h = nullptr;
+        size_t cap = 0;
+        hipEvent_t ev = nullptr;
+        bool pending = false;
+    } stage_P, stage_roots;
 };
 
 namespace {
@@ -628,6 +637,39 @@ hipError_t ensure(T*& ptr, size_t& have, size_t need) {
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&ptr), need ? need : 16);
     if (e == hipSuccess) have = need;
     return e;
+}
+
+// Host staging area of n doubles, free to overwrite once the copy that last
+// read it has completed.
+hipError_t stage_acquire(phylo_ctx::Stage& st, size_t n, double** out) {
+    hipError_t e;
+    if (st.pending) {
+        if ((e = hipEventSynchronize(st.ev)) != hipSuccess) return e;
+        st.pending = false;
+    }
+    if (!st.ev && (e = hipEventCreateWithFlags(&st.ev, hipEventDisableTiming)) != hipSuccess) return e;
+    if (st.cap < n) {
+        if (st.h && (e = hipHostFree(st.h)) != hipSuccess) return e;
+        st.h = nullptr;
+        st.cap = 0;
+        if ((e = hipHostMalloc(reinterpret_cast<void**>(&st.h), n * sizeof(double))) != hipSuccess) return e;
+        st.cap = n;
+    }
+    *out = st.h;
+    return hipSuccess;
+}
+
+hipError_t stage_release(phylo_ctx::Stage& st, hipStream_t stream) {
+    hipError_t e = hipEventRecord(st.ev, stream);
+    st.pending = e == hipSuccess;
+    return e;
+}
+
+void stage_free(phylo_ctx::Stage& st) {
+    if (st.pending && st.ev) (void)hipEventSynchronize(st.ev);
+    if (st.h) (void)hipHostFree(st.h);
+    if (st.ev) (void)hipEventDestroy(st.ev);
+    st = phylo_ctx::Stage{};
 }
 
 // States per stored partial: 4 for the DNA walker, else the next supported
@@ -716,15 +758,19 @@ int upload_pmatrices(phylo_ctx* c) {
     HIPCHK(c, ensure(c->d_P, c->P_bytes, nP * sizeof(double)));
     if (c->builder == PHYLO_P_EIGEN) {
         const int S = c->S_model;
-        std::vector<double> eig;
-        eig.insert(eig.end(), c->V.begin(), c->V.end());
-        eig.insert(eig.end(), c->Vinv.begin(), c->Vinv.end());
-        eig.insert(eig.end(), c->lam.begin(), c->lam.end());
-        eig.insert(eig.end(), c->cat_rate.begin(), c->cat_rate.end());
-        HIPCHK(c, ensure(c->d_eig, c->eig_bytes, eig.size() * sizeof(double)));
+        const size_t n_eig = 2 * c->V.size() + c->lam.size() + c->cat_rate.size();
+        double* h = nullptr;
+        HIPCHK(c, stage_acquire(c->stage_P, n_eig + c->eff.size(), &h));
+        double* w = std::copy(c->V.begin(), c->V.end(), h);
+        w = std::copy(c->Vinv.begin(), c->Vinv.end(), w);
+        w = std::copy(c->lam.begin(), c->lam.end(), w);
+        w = std::copy(c->cat_rate.begin(), c->cat_rate.end(), w);
+        std::copy(c->eff.begin(), c->eff.end(), w);
+        HIPCHK(c, ensure(c->d_eig, c->eig_bytes, n_eig * sizeof(double)));
         HIPCHK(c, ensure(c->d_eff, c->eff_bytes, c->eff.size() * sizeof(double)));
-        HIPCHK(c, hipMemcpyAsync(c->d_eig, eig.data(), eig.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->d_eff, c->eff.data(), c->eff.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_eig, h, n_eig * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_eff, h + n_eig, c->eff.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, stage_release(c->stage_P, c->stream));
         const double* dV = c->d_eig;
         const double* dVi = dV + (size_t)S * S;
         const double* dl = dVi + (size_t)S * S;
@@ -734,7 +780,6 @@ int upload_pmatrices(phylo_ctx* c) {
         hipLaunchKernelGGL(pbuild_eigen_kernel, dim3((unsigned)blocks), dim3(threads), 0, c->stream, dV, dVi, dl,
                            c->d_eff, dr, S, Sp, c->C, c->n_nodes, c->d_P);
         HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipStreamSynchronize(c->stream));  // host vectors above go out of scope
     } else {
         std::vector<double> P;
         host_pmatrices(c, Sp, P);
@@ -809,12 +854,15 @@ int prepare(phylo_ctx* c) {
         if (rc) return rc;
     }
     if (c->dirty_roots) {
-        std::vector<double> roots(kMaxStates + kMaxCat, 0.0);
+        const size_t n_roots = kMaxStates + kMaxCat;
+        double* roots = nullptr;
+        HIPCHK(c, stage_acquire(c->stage_roots, n_roots, &roots));
+        std::fill(roots, roots + n_roots, 0.0);
         for (int i = 0; i < c->S; ++i) roots[i] = c->root_w[i];
         for (int k = 0; k < c->C; ++k) roots[kRootCatOff + k] = c->cat_w[k];
-        if (!c->d_roots) HIPCHK(c, hipMalloc(&c->d_roots, roots.size() * sizeof(double)));
-        HIPCHK(c, hipMemcpyAsync(c->d_roots, roots.data(), roots.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (!c->d_roots) HIPCHK(c, hipMalloc(&c->d_roots, n_roots * sizeof(double)));
+        HIPCHK(c, hipMemcpyAsync(c->d_roots, roots, n_roots * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, stage_release(c->stage_roots, c->stream));
         c->dirty_roots = false;
     }
     return PHYLO_OK;
@@ -1080,6 +1128,8 @@ void phylo_destroy(phylo_ctx* c) {
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->h_total) (void)hipHostFree(c->h_total);
+    stage_free(c->stage_P);
+    stage_free(c->stage_roots);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1222,7 +1272,6 @@ int phylo_set_categories(phylo_ctx* c, int n_cat, const double* rates, const dou
     c->dirty_P = true;
     c->clv_valid = false;
     c->dirty_roots = true;
-    if (c->d_clv) { (void)hipFree(c->d_clv); c->d_clv = nullptr; c->clv_bytes = 0; }
     return PHYLO_OK;
 }
 
