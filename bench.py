@@ -53,7 +53,7 @@ def cpu_baseline(wl, budget_s=20.0):
     t = wl.tree
     n = t.n_tips
     harness = O.REF_HARNESS
-    m_sample = 400
+    m_sample = 1200
     cores = max(1, min(16, os.cpu_count() or 1))
     if os.path.exists(harness) and os.access(harness, os.X_OK):
         with tempfile.TemporaryDirectory() as d:
@@ -98,6 +98,17 @@ def cpu_baseline(wl, budget_s=20.0):
             "sample": f"oracle restatement of simpleML's cost model, {n} taxa x {m_sample} sites, 1 category"}
 
 
+def profiled_traffic(name):
+    """HBM bytes per launch of a kernel on its bench workload, from the
+    FETCH_SIZE / WRITE_SIZE counter passes summarised in profiles/<name>.md
+    (scaled by a copy kernel of known size); None if not profiled."""
+    tf = os.path.join(ROOT, "profiles", name + "_traffic.json")
+    try:
+        return float(json.load(open(tf))["bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def bench_protein(args, P, synth):
     """20-state protein, 64-taxon Yule tree, 20,000 sites, 190 exchangeabilities
     ~ Gamma(1,1), frequencies ~ Dirichlet(2x20), gamma shape 1.0 (4 categories),
@@ -134,7 +145,8 @@ def bench_protein(args, P, synth):
             "config": {"workload": "20-state protein + G4, 64 taxa Yule, 20000 sites, 3% gaps", "n_taxa": n,
                        "n_sites": m, "n_states": S, "n_cat": C, "clv_storage": "f32", "lnL": lnl},
             "roofline": {"bound": "hbm", "achieved": bpu * updates / ks / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": bpu * updates / ks / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                         "frac": bpu * updates / ks / 1e9 / HBM_PEAK_GBS,
+                         "traffic": profiled_traffic("prune_wide_protein20_g4_64x20k"),
                          "kernel": "prune_wide_kernel<float,4,20>", "kernel_ms": ks * 1e3,
                          "alg_bytes_per_update": bpu,
                          "fp64_tflops": flops * updates / ks / 1e12, "fp64_peak_tflops": 78.6}}
@@ -278,13 +290,7 @@ def main():
     kernel_s = float(np.mean(kms)) * 1e-3
     bpu = alg_bytes_per_update(n, 4, C) * (2 if args.fp64 else 1)
     achieved = bpu * updates / kernel_s / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "prune4_gtr_g4_128x100k_traffic.json")
-    if os.path.exists(tf) and not args.fp64 and m == 100_000 and n == 128:
-        try:
-            traffic = json.load(open(tf)).get("bytes_per_launch")
-        except Exception:
-            traffic = None
+    traffic = profiled_traffic("prune4_gtr_g4_128x100k") if (not args.fp64 and m == 100_000 and n == 128) else None
     out = {
         "metric": "CLV updates/s (site*node*cat)",
         "value": value,
