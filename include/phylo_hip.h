@@ -126,6 +126,19 @@ int phylo_get_pmatrix(phylo_ctx *ctx, int node, double *P);
 int phylo_nni_score(phylo_ctx *ctx, int mode, int max_variants, double *lnl, int32_t *node, int32_t *side,
                     int *n_variants, double *base_lnl);
 
+/* Forward simulation on the GPU, replacing simpleML::simulate_chars
+ * (simpleML.cpp:233-292): one thread per site draws a rate category from the
+ * category weights, the root state from root_freq (n_states values) and then
+ * every node's state from row P[parent state] of its edge's P(t) for that
+ * category, with the reference's subtract-until-non-positive inverse CDF.
+ * Uses the context's topology, edge lengths, model / P builder and
+ * categories (phylo_set_tips fixes the numbering; its tip data is unused).
+ * states: n_nodes x n_sites bytes, [node][site].  Draws come from a
+ * counter-based generator keyed by (seed, site, node), so results do not
+ * depend on the launch shape; the reference draws from rand() seeded by the
+ * clock and cannot be reproduced. */
+int phylo_simulate(phylo_ctx *ctx, int n_sites, const double *root_freq, uint64_t seed, uint8_t *states);
+
 /* Number of partial-recompute ops the last phylo_nni_score ran (updates
  * per site and category), for throughput accounting. */
 int phylo_nni_ops(phylo_ctx *ctx, long *n_ops);

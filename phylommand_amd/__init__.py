@@ -59,6 +59,7 @@ SIGNATURES = [
                                    ctypes.POINTER(ctypes.c_int)]),
     ("phylo_batch_destroy", None, [_c_void_p]),
     ("phylo_last_kernel_ms", _c_int, [_c_void_p, _c_dbl_p]),
+    ("phylo_simulate", _c_int, [_c_void_p, _c_int, _c_dbl_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint8)]),
     # include/phylo_learn.h
     ("phylo_sim_features", _c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int_p, _c_int_p, _c_int_p, _c_dbl_p,
                                     _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.c_double, ctypes.c_uint64, _c_flt_p,
@@ -288,6 +289,14 @@ class Context:
         ms = np.zeros(1)
         self._check(self.lib.phylo_last_kernel_ms(self.h, _dp(ms)))
         return float(ms[0])
+
+    def simulate(self, n_sites: int, root_freq, seed: int = 1) -> np.ndarray:
+        """States [n_nodes, n_sites] drawn down the context's tree and model."""
+        rf = np.ascontiguousarray(root_freq, dtype=np.float64)
+        out = np.zeros((self.n_nodes, n_sites), np.uint8)
+        self._check(self.lib.phylo_simulate(self.h, int(n_sites), _dp(rf), int(seed),
+                                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        return out
 
     def gtr_gamma_lnl(self, rel_len, n_cat: int, x) -> float:
         """lnL of the GTR+Gamma point x (see include/phylo_host.h)."""

@@ -543,6 +543,60 @@ __global__ void pbuild_eigen_kernel(const double* __restrict__ V, const double* 
     P[gid] = v;
 }
 
+__device__ inline uint64_t sim_mix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+// Uniform in [0, 1) with 53 random bits from the counter (seed, site, stream).
+__device__ inline double sim_uniform(uint64_t seed, uint64_t site, uint64_t stream) {
+    const uint64_t h = sim_mix(seed ^ sim_mix(site * 0x9e3779b97f4a7c15ull ^ sim_mix(stream + 0x632be59bd9b4e019ull)));
+    return (double)(h >> 11) * 0x1.0p-53;
+}
+
+// Forward simulation, one thread per site (simpleML::simulate_chars,
+// simpleML.cpp:233-292): a rate category from the category weights, the root
+// state from root_cum by the reference's subtract-until-non-positive walk
+// (state 0 when it never gets there), then down the preorder each node's
+// state from row P[parent state] the same way (the parent's state when the
+// walk does not end).  States go to out[node][site].
+__global__ __launch_bounds__(256) void simulate_kernel(const int* __restrict__ pre, const int* __restrict__ parent,
+                                                       int n_nodes, int m, int S, int Sp, int C,
+                                                       const double* __restrict__ P, const double* __restrict__ root_f,
+                                                       const double* __restrict__ cat_w, uint64_t seed,
+                                                       uint8_t* __restrict__ out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m) return;
+    int cat = 0;
+    if (C > 1) {
+        double u = sim_uniform(seed, s, 0);
+        cat = C - 1;
+        for (int k = 0; k < C; ++k) {
+            u -= cat_w[k];
+            if (u < 0.0) { cat = k; break; }
+        }
+    }
+    double u = sim_uniform(seed, s, 1);
+    int st = 0;
+    for (int j = 0; j < S; ++j) {
+        u -= root_f[j];
+        if (u <= 0.0) { st = j; break; }
+    }
+    out[(size_t)pre[0] * m + s] = (uint8_t)st;
+    for (int k = 1; k < n_nodes; ++k) {
+        const int v = pre[k];
+        const int anc = out[(size_t)parent[v] * m + s];
+        const double* row = P + (((size_t)v * C + cat) * Sp + anc) * Sp;
+        double r = sim_uniform(seed, s, 2 + (uint64_t)v);
+        int x = anc;
+        for (int j = 0; j < S; ++j) {
+            r -= row[j];
+            if (r <= 0.0) { x = j; break; }
+        }
+        out[(size_t)v * m + s] = (uint8_t)x;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1379,6 +1433,64 @@ int phylo_nni_score(phylo_ctx* c, int mode, int max_variants, double* lnl, int32
     if (rc) return rc;
     for (int q = 0; q < n_prog; ++q) lnl[which[q]] = tot[q];
     c->nni_ops_last = (long)ops.size();
+    return PHYLO_OK;
+}
+
+int phylo_simulate(phylo_ctx* c, int n_sites, const double* root_freq, uint64_t seed, uint8_t* states) {
+    if (!c) return PHYLO_ERR_ARG;
+    if (n_sites < 1 || !root_freq || !states) return fail(c, PHYLO_ERR_ARG, "simulate: bad arguments");
+    if (!c->have_topo) return fail(c, PHYLO_ERR_STATE, "simulate: phylo_set_topology has not been called");
+    if (!c->have_eff) return fail(c, PHYLO_ERR_STATE, "simulate: phylo_set_edge_lengths has not been called");
+    if (!c->have_model) return fail(c, PHYLO_ERR_STATE, "simulate: phylo_set_model has not been called");
+    const int S = c->S_model, n = c->n_nodes;
+    for (int j = 0; j < S; ++j)
+        if (!std::isfinite(root_freq[j])) return fail(c, PHYLO_ERR_ARG, "simulate: non-finite root frequency");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->dirty_P) {
+        int rc = upload_pmatrices(c);
+        if (rc) return rc;
+    }
+    // preorder and parents on the host (every node reachable: checked by set_topology)
+    std::vector<int> pre, parent(n, -1), stack{c->root};
+    pre.reserve(n);
+    while (!stack.empty()) {
+        const int v = stack.back();
+        stack.pop_back();
+        pre.push_back(v);
+        if (c->right[v] >= 0) { parent[c->right[v]] = v; stack.push_back(c->right[v]); }
+        if (c->left[v] >= 0) { parent[c->left[v]] = v; stack.push_back(c->left[v]); }
+    }
+    if ((int)pre.size() != n) return fail(c, PHYLO_ERR_ARG, "simulate: nodes unreachable from the root");
+    double wsum = 0.0;
+    for (int k = 0; k < c->C; ++k) wsum += c->cat_w[k];
+    std::vector<double> aux(S + c->C);
+    for (int j = 0; j < S; ++j) aux[j] = root_freq[j];
+    for (int k = 0; k < c->C; ++k) aux[S + k] = wsum > 0 ? c->cat_w[k] / wsum : 1.0 / c->C;
+    int* d_idx = nullptr;
+    double* d_aux = nullptr;
+    uint8_t* d_out = nullptr;
+    const size_t n_out = (size_t)n * n_sites;
+    auto cleanup = [&]() {
+        if (d_idx) (void)hipFree(d_idx);
+        if (d_aux) (void)hipFree(d_aux);
+        if (d_out) (void)hipFree(d_out);
+    };
+    std::vector<int> idx(pre);
+    idx.insert(idx.end(), parent.begin(), parent.end());
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_idx), idx.size() * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&d_aux), aux.size() * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&d_out), n_out);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_aux, aux.data(), aux.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(simulate_kernel, dim3((unsigned)((n_sites + 255) / 256)), dim3(256), 0, c->stream, d_idx,
+                           d_idx + n, n, n_sites, S, padded_states(S), c->C, c->d_P, d_aux, d_aux + S, seed, d_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(states, d_out, n_out, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    cleanup();
+    if (e != hipSuccess) return fail(c, PHYLO_ERR_HIP, std::string("simulate: ") + hipGetErrorString(e));
     return PHYLO_OK;
 }
 

@@ -209,19 +209,57 @@ void simpleML::draw_normalized_likelihood_on_nodes() {
 }
 
 // simulate_chars (simpleML.cpp:233-292): a root state drawn from charfreq,
-// then on every edge a state drawn from row P[ancestor] of P(branch length).
+// then on every edge a state drawn from row P[ancestor] of P(raw branch
+// length), on the GPU (phylo_simulate).  Rows are returned for every labelled
+// non-root node in preorder, as the reference does.
 std::vector<character_vector> simpleML::simulate_chars(const std::vector<double>& charfreq, const unsigned int n_char,
                                                        sub_model& model) {
-    std::mt19937_64 rng(seeded_ ? seed_ : std::random_device{}());
-    std::uniform_real_distribution<double> U(0.0, 1.0);
+    std::vector<character_vector> out;
+    if (!ctx_ && phylo_create(0, &ctx_)) {
+        err_ = "no GPU available for libphylo_hip";
+        std::cerr << err_ << std::endl;
+        return out;
+    }
     const unsigned int S = model.get_n_states();
-    std::vector<std::vector<double>> P(nodes_.size());
+    const std::vector<int> tip = tips();
+    if (n_char == 0 || tip.size() < 2) return out;
+    std::vector<int> lib(nodes_.size(), -1);
+    int next = 0;
+    for (int v : tip) lib[v] = next++;
+    for (size_t v = 0; v < nodes_.size(); ++v)
+        if (!is_tip((int)v)) lib[v] = next++;
+    const int n_lib = next;
+    std::vector<int32_t> left(n_lib, -1), right(n_lib, -1);
+    std::vector<double> len(n_lib, 0.0);
     for (size_t v = 0; v < nodes_.size(); ++v) {
-        if ((int)v == root_) continue;
-        model.set_P_matrix(nodes_[v].branchlength);
-        P[v].resize((size_t)S * S);
-        for (unsigned int i = 0; i < S; ++i)
-            for (unsigned int j = 0; j < S; ++j) P[v][(size_t)i * S + j] = model.get_P_value(i, j);
+        len[lib[v]] = nodes_[v].branchlength;
+        if (is_tip((int)v)) continue;
+        const int l = nodes_[v].left, r = nodes_[v].right;
+        left[lib[v]] = l >= 0 ? lib[l] : (r >= 0 ? lib[r] : -1);
+        right[lib[v]] = (l >= 0 && r >= 0) ? lib[r] : -1;
+    }
+    // the tip data is not used by the simulation; one all-state site fixes the numbering
+    const std::vector<uint32_t> any(tip.size(), S >= 32 ? 0xFFFFFFFFu : ((1u << S) - 1u));
+    const std::vector<double> pi = model.frequencies();
+    const std::vector<double> rw = model.root_weights();
+    int builder = builder_;
+    if (builder == P_AUTO) builder = model.is_tr() ? P_EIGEN : P_PADE;
+    std::vector<double> root_f(S, 0.0);
+    for (unsigned int j = 0; j < S && j < charfreq.size(); ++j) root_f[j] = charfreq[j];
+    std::vector<uint8_t> states((size_t)n_lib * n_char);
+    int rc = phylo_set_tips(ctx_, (int)tip.size(), 1, (int)S, any.data());
+    if (!rc) rc = phylo_set_topology(ctx_, n_lib, lib[root_], left.data(), right.data());
+    if (!rc) rc = phylo_set_edge_lengths(ctx_, len.data());
+    if (!rc) rc = phylo_set_model(ctx_, (int)S, model.Q().data(), rw.data(), builder == P_EIGEN ? pi.data() : nullptr,
+                                  builder);
+    if (!rc) rc = phylo_set_categories(ctx_, (int)cat_rates_.size(), cat_rates_.data(), cat_w_.data());
+    if (!rc) rc = phylo_simulate(ctx_, (int)n_char, root_f.data(), seeded_ ? seed_ : std::random_device{}(),
+                                 states.data());
+    lib_of_node_.clear();       // the context no longer holds this tree's likelihood inputs
+    if (rc) {
+        err_ = phylo_last_error(ctx_);
+        std::cerr << err_ << std::endl;
+        return out;
     }
     std::vector<int> pre, stack{root_};
     while (!stack.empty()) {
@@ -231,37 +269,18 @@ std::vector<character_vector> simpleML::simulate_chars(const std::vector<double>
         if (nodes_[v].right >= 0) stack.push_back(nodes_[v].right);
         if (nodes_[v].left >= 0) stack.push_back(nodes_[v].left);
     }
-    std::vector<character_vector> rows(nodes_.size());
-    std::vector<unsigned int> st(nodes_.size(), 0);
-    for (unsigned int s = 0; s < n_char; ++s) {
-        double u = U(rng);
-        unsigned int root_state = 0;
-        for (unsigned int j = 0; j < charfreq.size(); ++j) {
-            u -= charfreq[j];
-            if (u <= 0) { root_state = j; break; }
+    for (int v : pre) {
+        if (v == root_ || !nodes_[v].has_label || nodes_[v].label.empty()) continue;
+        character_vector row;
+        const uint8_t* st = states.data() + (size_t)lib[v] * n_char;
+        for (unsigned int c = 0; c < n_char; ++c) {
+            state_set x;
+            x.set(st[c]);
+            row.add_character(x);
         }
-        st[root_] = root_state;
-        for (int v : pre) {
-            if (v == root_) continue;
-            const unsigned int anc = st[nodes_[v].parent];
-            double r = U(rng);
-            unsigned int trait = anc;
-            for (unsigned int i = 0; i < S; ++i) {
-                r -= P[v][(size_t)anc * S + i];
-                if (r <= 0) { trait = i; break; }
-            }
-            st[v] = trait;
-            state_set c;
-            c.set(trait);
-            rows[v].add_character(c);
-        }
+        row.set_taxon(nodes_[v].label);
+        out.push_back(row);
     }
-    std::vector<character_vector> out;
-    for (int v : pre)
-        if (v != root_ && nodes_[v].has_label && !nodes_[v].label.empty()) {
-            rows[v].set_taxon(nodes_[v].label);
-            out.push_back(rows[v]);
-        }
     return out;
 }
 

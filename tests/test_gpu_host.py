@@ -109,3 +109,40 @@ def test_host_nni_scores_match_reevaluation(tmp_path):
         _, tot = parse_out(drive("persite", p, *args[1:]))
         # the variant tree was printed with 6-decimal lengths
         assert abs(tot - scores[k]) <= 1e-3 * max(1.0, abs(tot)) + 0.05, (k, tot, scores[k])
+
+
+def test_host_simulate_chars_pattern_frequencies(tmp_path, oracle):
+    """simulate_chars on the GPU through the host class (parity unpinned: the
+    reference draws from time-seeded rand()): the counts of the 256 site
+    patterns of a 4-taxon tree against exp(site lnL) of the oracle with the
+    same raw branch lengths and root frequencies (chi-square)."""
+    from scipy.stats import chi2
+    print("host simulate_chars: 4 taxa x 200000 sites", flush=True)
+    p = tmp_path / "t.nwk"
+    p.write_text("((a:0.3,b:0.2):0.1,(c:0.5,d:0.05):0.2);\n")
+    m = 200_000
+    out = drive("simulate", p, "dna", 1, "0,1,0,0,1,0", "1.0,0.3", "0.1,0.2,0.3,0.4", m, 7)
+    lines = out.strip().split("\n")
+    names, rows = [x[1:] for x in lines[0::2]], lines[1::2]
+    assert sorted(names) == ["a", "b", "c", "d"] and all(len(r) == m for r in rows)
+    code = {ch: i for i, ch in enumerate("AGCT")}
+    st = {n: np.frombuffer(r.encode(), np.uint8) for n, r in zip(names, rows)}
+    lut = np.zeros(256, np.int64)
+    for ch, i in code.items():
+        lut[ord(ch)] = i
+    pat = sum(lut[st[n]] * 4 ** (3 - k) for k, n in enumerate("abcd"))
+    counts = np.bincount(pat, minlength=256)
+    # oracle: tips a,b,c,d = nodes 0..3, internals 4 (ab), 5 (cd), root 6
+    left = np.array([-1, -1, -1, -1, 0, 2, 4], np.int32)
+    right = np.array([-1, -1, -1, -1, 1, 3, 5], np.int32)
+    blen = np.array([0.3, 0.2, 0.5, 0.05, 0.1, 0.2, 0.0])
+    Q = oracle.build_Q(4, 1, [0, 1, 0, 0, 1, 0], [1.0, 0.3], [0.1, 0.2, 0.3, 0.4])
+    pi = np.array([0.1, 0.2, 0.3, 0.4])
+    pats = np.array([[(q >> (2 * (3 - t))) & 3 for q in range(256)] for t in range(4)])
+    mb = np.zeros((7, 256), np.uint32)
+    mb[:4] = 1 << pats
+    _, site = oracle.Problem(left, right, 6, blen, mb, 4, Q, pi, pi=pi, builder=oracle.EIG).loglik()
+    exp = np.exp(site) * m
+    keep = exp >= 5
+    stat = np.sum((counts[keep] - exp[keep]) ** 2 / exp[keep])
+    assert stat < chi2.ppf(1 - 1e-6, keep.sum() - 1), stat

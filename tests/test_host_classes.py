@@ -53,15 +53,3 @@ def test_pmatrix_bit_exact():
         vals = np.array([[float(x) for x in line.split()] for line in out.strip().split("\n")])
         want = np.array(c["P"]).reshape(-1, c["S"])
         np.testing.assert_array_equal(vals, want, err_msg=key)
-
-
-def test_simulation_frequencies(tmp_path):
-    """simulate_chars (parity unpinned: the reference draws from time-seeded
-    rand()): long simulations reproduce the root frequencies at the tips."""
-    p = tmp_path / "t.nwk"
-    p.write_text("((a:0.3,b:0.2):0.1,(c:0.5,d:0.05):0.2);\n")
-    out = drive("simulate", p, "dna", 1, "0,1,0,0,1,0", "1.0,0.3", "0.1,0.2,0.3,0.4", 20000, 7)
-    rows = out.strip().split("\n")[1::2]
-    assert len(rows) == 4 and all(len(r) == 20000 for r in rows)
-    freq = np.array([sum(r.count(ch) for r in rows) for ch in "AGCT"]) / 80000
-    np.testing.assert_allclose(freq, [0.1, 0.2, 0.3, 0.4], atol=0.02)
