@@ -319,178 +319,179 @@ __global__ __launch_bounds__(64) void segment_sum_kernel(const double* __restric
     out[k] = s;
 }
 
-// One child of the wide-state walker: acc[i] (=|*=) sum_j P[i][j] x_j.
-// Stored/carried partials use the row-major P (wave-uniform, scalar loads);
-// tips gather columns of the transposed copy PT (a single state: one column;
-// all states: the row-sum column S; an ambiguity set: the sum of its columns).
-template <typename T, int C, int SP>
-__device__ inline void wide_child(int kind, int idx, int node, int cat, int s, int m_pad, int S, uint32_t full,
-                                  const uint32_t* __restrict__ tips, const T* plane, size_t slot_stride,
-                                  const int* scl, const double* __restrict__ P, const double* __restrict__ PT,
-                                  const T (&cA)[SP], int sA, const T (&cB)[SP], int sB, double (&acc)[SP],
-                                  int& sc, bool mul) {
+// ---------------------------------------------------------------------------
+// Wide-state walker on the matrix cores (4 < S <= 32, FP64 arithmetic).
+//
+// One wave = 16 sites x one rate category; a workgroup holds the C category
+// waves of one 16-site group.  Each child product is a GEMM
+//     out[state row][site] = sum_k P[row][k] * L[k][site]
+// done as v_mfma_f64_16x16x4f64 steps (A = P: 16 rows x 4 k, B = L: 4 k x 16
+// sites, D = 16 rows x 16 sites; D[i][j] sits in lane j + 16 (i % 4),
+// register i / 4).  Lane l serves site l % 16 and lane group g = l / 16; slot
+// t of a lane holds state 4t + g, and row i of tile T is state 16T + i, so an
+// MFMA result (tile T, register r) is slot 4T + r of the next op's operand
+// with no shuffles.  V = ceil(S / 4) slots cover the S states; tile 1 exists
+// for S > 16.  Partials are stored as [slot][cat][site group][V][64 lanes]
+// (every load and store a full row), P as [node][cat][tile][V][64] doubles
+// (pmfma_kernel).  FP64 matrix cores rather than FP32 ones: the FP32 MFMA
+// dot products truncate small addends (about -0.1 ulp per 5-step chain on
+// transition-matrix-shaped operands), which biases every site the same way.
+// Scalers as in the other walkers: one int32 power of two per (node, site)
+// shared by the categories, exchanged through LDS with one barrier per op.
+// ---------------------------------------------------------------------------
+
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ constexpr int mfma_slots(int S) { return (S + 3) / 4; }
+
+// Pm[node][cat][T][t][lane] = P[16T + lane % 16][4t + lane / 16] (0 outside S).
+__global__ void pmfma_kernel(const double* __restrict__ P, int S, int Sp, int C, int n_nodes, int V, int TN,
+                             double* __restrict__ Pm) {
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)n_nodes * C * TN * V * 64;
+    if (gid >= total) return;
+    const int l = gid % 64;
+    const int t = (gid / 64) % V;
+    const int T = (gid / (64L * V)) % TN;
+    const long mat = gid / (64L * V * TN);          // node * C + cat
+    const int row = 16 * T + l % 16, col = 4 * t + l / 16;
+    Pm[gid] = (row < S && col < S) ? P[(mat * Sp + row) * Sp + col] : 0.0;
+}
+
+__device__ inline double to_d(float x) { return (double)x; }
+__device__ inline double to_d(double x) { return x; }
+
+// One child: acc[T] = P . x for tile T, x in operand order.
+template <typename T, int C, int V>
+__device__ inline void mfma_child(int kind, int idx, int node, int cat, int s, int sg, int n_sg, int lane, int g,
+                                  int m_pad, const uint32_t* __restrict__ tips, const T* clv, size_t slot_stride,
+                                  const int* scl, const double* __restrict__ Pm, const double (&cA)[V], int sA,
+                                  const double (&cB)[V], int sB, mfma_d4 (&acc)[2], int& sc) {
+    constexpr int TN = V > 4 ? 2 : 1;
+    double x[V];
     if (kind == SRC_TIP) {
         const uint32_t mk = tips[(size_t)idx * m_pad + s];
-        const double* cols = PT + ((size_t)node * C + cat) * (size_t)(S + 1) * SP;
-        double out[SP];
-        if (mk == full || __popc(mk) == 1) {
-            const int j = mk == full ? S : __ffs(mk) - 1;
-            const double2* col = reinterpret_cast<const double2*>(cols + (size_t)j * SP);
 #pragma unroll
-            for (int i = 0; i < SP / 2; ++i) {
-                const double2 v = col[i];
-                out[2 * i] = v.x;
-                out[2 * i + 1] = v.y;
-            }
-        } else {
+        for (int t = 0; t < V; ++t) x[t] = (mk >> (4 * t + g)) & 1u ? 1.0 : 0.0;
+    } else if (kind == SRC_CLV) {
+        const T* src = clv + (size_t)idx * slot_stride + ((size_t)cat * n_sg + sg) * V * 64 + lane;
 #pragma unroll
-            for (int i = 0; i < SP; ++i) out[i] = 0.0;
-            uint32_t rest = mk;
-            while (rest) {
-                const int j = __ffs(rest) - 1;
-                rest &= rest - 1;
-                const double2* col = reinterpret_cast<const double2*>(cols + (size_t)j * SP);
-#pragma unroll
-                for (int i = 0; i < SP / 2; ++i) {
-                    const double2 v = col[i];
-                    out[2 * i] += v.x;
-                    out[2 * i + 1] += v.y;
-                }
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < SP; ++i) acc[i] = mul ? acc[i] * out[i] : out[i];
-        return;
-    }
-    double x[SP];
-    if (kind == SRC_CLV) {
-        const T* src = plane + (size_t)idx * slot_stride;
-#pragma unroll
-        for (int q = 0; q < SP / 4; ++q) load4(src + 4 * q, x + 4 * q);
+        for (int t = 0; t < V; ++t) x[t] = to_d(src[t * 64]);
         sc += scl[(size_t)idx * m_pad + s];
     } else if (kind == SRC_CARRY_A) {
 #pragma unroll
-        for (int j = 0; j < SP; ++j) x[j] = (double)cA[j];
+        for (int t = 0; t < V; ++t) x[t] = cA[t];
         sc += sA;
     } else {
 #pragma unroll
-        for (int j = 0; j < SP; ++j) x[j] = (double)cB[j];
+        for (int t = 0; t < V; ++t) x[t] = cB[t];
         sc += sB;
     }
-    const double* Pn = P + ((size_t)node * C + cat) * SP * SP;
+    const double* pm = Pm + ((size_t)node * C + cat) * TN * V * 64 + lane;
 #pragma unroll
-    for (int i = 0; i < SP; ++i) {
-        double a = 0.0;
+    for (int T2 = 0; T2 < TN; ++T2) {
+        acc[T2] = mfma_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int j = 0; j < SP; ++j) a = fma(Pn[i * SP + j], x[j], a);
-        acc[i] = mul ? acc[i] * a : a;
+        for (int t = 0; t < V; ++t)
+            acc[T2] = __builtin_amdgcn_mfma_f64_16x16x4f64(pm[(T2 * V + t) * 64], x[t], acc[T2], 0, 0, 0);
     }
 }
 
-// The pruning walker for 4 < S <= 32 (protein and other wide alphabets):
-// the DNA walker's structure with S padded to SP (a multiple of 4) states
-// per partial, 32-bit tip masks, and FP64 arithmetic on unpadded-in-effect
-// rows (padded rows and columns of P are zero).
-template <typename T, int C, int SP>
-__global__ __launch_bounds__(64 * C) void prune_wide_kernel(
-    const Op* __restrict__ ops, const int* __restrict__ prog_begin, const uint32_t* __restrict__ tips,
-    T* clv, int* scl, const double* __restrict__ P, const double* __restrict__ PT, const double* __restrict__ roots,
-    const double* __restrict__ site_w, double* __restrict__ site_lnl, double* __restrict__ partial,
-    int m, int m_pad, int S, uint32_t full) {
-    __shared__ int xexp[2][C][64];
-    __shared__ double xroot[C][64];
+// grid = (m_pad / 16, n_programs), block = 64 * C.
+template <typename T, int C, int V>
+__global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
+    const Op* __restrict__ ops, const int* __restrict__ prog_begin, const uint32_t* __restrict__ tips, T* clv,
+    int* scl, const double* __restrict__ Pm, const double* __restrict__ roots, const double* __restrict__ site_w,
+    double* __restrict__ site_lnl, double* __restrict__ partial, int m, int m_pad) {
+    constexpr int TN = V > 4 ? 2 : 1;
+    __shared__ int xexp[2][C][16];
+    __shared__ double xroot[C][16];
     const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, j = lane & 15;
     const int cat = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int s = blockIdx.x * 64 + lane;
+    const int sg = blockIdx.x, n_sg = m_pad >> 4;
+    const int s = sg * 16 + j;
     const int prog = blockIdx.y;
     const int k0 = prog_begin[prog], k1 = prog_begin[prog + 1];
-    T cA[SP], cB[SP];
+    const size_t slot_stride = (size_t)C * m_pad * 4 * V;
+    double cA[V], cB[V];
 #pragma unroll
-    for (int i = 0; i < SP; ++i) { cA[i] = T(0); cB[i] = T(0); }
+    for (int t = 0; t < V; ++t) { cA[t] = 0.0; cB[t] = 0.0; }
     int sA = 0, sB = 0;
     double lnl = 0.0;
-    T* plane = clv + ((size_t)cat * m_pad + s) * SP;
-    const size_t slot_stride = (size_t)C * m_pad * SP;
     for (int k = k0; k < k1; ++k) {
         const Op op = ops[k];
-        double acc[SP];
         int sc = 0;
-        wide_child<T, C, SP>(op.l_kind, op.l_idx, op.l_node, cat, s, m_pad, S, full, tips, plane, slot_stride, scl,
-                             P, PT, cA, sA, cB, sB, acc, sc, false);
-        if (op.r_kind != SRC_NONE)
-            wide_child<T, C, SP>(op.r_kind, op.r_idx, op.r_node, cat, s, m_pad, S, full, tips, plane, slot_stride,
-                                 scl, P, PT, cA, sA, cB, sB, acc, sc, true);
+        mfma_d4 la[2], ra[2];
+        mfma_child<T, C, V>(op.l_kind, op.l_idx, op.l_node, cat, s, sg, n_sg, lane, g, m_pad, tips, clv, slot_stride,
+                            scl, Pm, cA, sA, cB, sB, la, sc);
+        double val[V];
+        if (op.r_kind != SRC_NONE) {
+            mfma_child<T, C, V>(op.r_kind, op.r_idx, op.r_node, cat, s, sg, n_sg, lane, g, m_pad, tips, clv,
+                                slot_stride, scl, Pm, cA, sA, cB, sB, ra, sc);
+#pragma unroll
+            for (int t = 0; t < V; ++t) val[t] = la[t / 4][t % 4] * ra[t / 4][t % 4];
+        } else {
+#pragma unroll
+            for (int t = 0; t < V; ++t) val[t] = la[t / 4][t % 4];
+        }
+        (void)TN;
+        // per-site maximum over the lane groups, then over the category waves
         double mx = 0.0;
 #pragma unroll
-        for (int i = 0; i < SP; ++i) mx = fmax(mx, acc[i]);
+        for (int t = 0; t < V; ++t) mx = fmax(mx, val[t]);
+        mx = fmax(mx, __shfl_xor(mx, 16, 64));
+        mx = fmax(mx, __shfl_xor(mx, 32, 64));
         int e = INT_MIN;
         if (mx > 0.0) (void)frexp(mx, &e);
         if (C > 1) {
             const int buf = k & 1;
-            xexp[buf][cat][lane] = e;
+            if (g == 0) xexp[buf][cat][j] = e;
             __syncthreads();
 #pragma unroll
-            for (int c = 0; c < C; ++c) e = max(e, xexp[buf][c][lane]);
+            for (int c = 0; c < C; ++c) e = max(e, xexp[buf][c][j]);
         }
         if (e == INT_MIN) e = 0;
         sc += e;
-        T r[SP];
 #pragma unroll
-        for (int i = 0; i < SP; ++i) {
-            acc[i] = ldexp(acc[i], -e);
-            r[i] = (T)acc[i];
-        }
+        for (int t = 0; t < V; ++t) val[t] = to_d((T)ldexp(val[t], -e));   // stored precision, carried too
         if (op.out_slot >= 0) {
-            T* dst = plane + (size_t)op.out_slot * slot_stride;
+            T* dst = clv + (size_t)op.out_slot * slot_stride + ((size_t)cat * n_sg + sg) * V * 64 + lane;
 #pragma unroll
-            for (int q = 0; q < SP / 4; ++q) store4(dst + 4 * q, r + 4 * q);
-            if (cat == 0) scl[(size_t)op.out_slot * m_pad + s] = sc;
+            for (int t = 0; t < V; ++t) dst[t * 64] = (T)val[t];
+            if (cat == 0 && g == 0) scl[(size_t)op.out_slot * m_pad + s] = sc;
         }
         if (op.flags & OP_TO_B) {
 #pragma unroll
-            for (int i = 0; i < SP; ++i) cB[i] = r[i];
+            for (int t = 0; t < V; ++t) cB[t] = val[t];
             sB = sc;
         } else {
 #pragma unroll
-            for (int i = 0; i < SP; ++i) cA[i] = r[i];
+            for (int t = 0; t < V; ++t) cA[t] = val[t];
             sA = sc;
         }
         if (op.flags & OP_ROOT) {
             double lc = 0.0;
 #pragma unroll
-            for (int i = 0; i < SP; ++i) lc = fma(roots[i], acc[i], lc);
+            for (int t = 0; t < V; ++t) lc = fma(roots[4 * t + g], val[t], lc);
+            lc += __shfl_xor(lc, 16, 64);
+            lc += __shfl_xor(lc, 32, 64);
             lc *= roots[kRootCatOff + cat];
             if (C > 1) {
-                xroot[cat][lane] = lc;
+                if (g == 0) xroot[cat][j] = lc;
                 __syncthreads();
                 lc = 0.0;
 #pragma unroll
-                for (int c = 0; c < C; ++c) lc += xroot[c][lane];
+                for (int c = 0; c < C; ++c) lc += xroot[c][j];
                 __syncthreads();
             }
             lnl = log(lc) + sc * kLn2;
         }
     }
-    const bool live = s < m && cat == 0;
+    const bool live = s < m && cat == 0 && g == 0;
     if (live && site_lnl) site_lnl[(size_t)prog * m + s] = lnl;
     const double w = live ? (site_w ? site_w[s] : 1.0) : 0.0;
     block_sum_store(live ? w * lnl : 0.0, partial + (size_t)prog * gridDim.x + blockIdx.x);
-}
-
-// PT[node][cat][j][i] = P[node][cat][i][j] for j < S, PT[..][S][i] = sum_j P[i][j].
-__global__ void ptranspose_kernel(const double* __restrict__ P, int S, int SP, long n_mat, double* __restrict__ PT) {
-    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long per = (long)(S + 1) * SP;
-    if (gid >= n_mat * per) return;
-    const long mat = gid / per;
-    const int j = (gid % per) / SP;
-    const int i = gid % SP;
-    const double* Pm = P + mat * SP * SP;
-    double v = 0.0;
-    if (j < S) v = Pm[i * SP + j];
-    else
-        for (int q = 0; q < S; ++q) v += Pm[i * SP + q];
-    PT[gid] = v;
 }
 
 // Compensated fixed-order sum of the per-workgroup partials of each program
@@ -639,7 +640,7 @@ struct phylo_ctx {
 
     // device buffers
     void* d_tips = nullptr; size_t tips_bytes = 0;     // uint8 codes (S <= 4) or uint32 masks
-    double* d_PT = nullptr; size_t PT_bytes = 0;       // transposed P + row sums (wide states)
+    double* d_Pm = nullptr; size_t Pm_bytes = 0;       // P in MFMA operand order (wide states)
     void* d_clv = nullptr; size_t clv_bytes = 0;
     int* d_scl = nullptr; size_t scl_bytes = 0;
     double* d_P = nullptr; size_t P_bytes = 0;
@@ -736,6 +737,11 @@ int padded_states(int S) {
     if (S <= 24) return 24;
     return 32;
 }
+
+// States per stored partial site: the MFMA walker (S > 4) keeps 4 * V operand
+// slots per site, the DNA walker 4.
+int clv_states(const phylo_ctx* c) { return c->S > 4 ? 4 * mfma_slots(c->S) : 4; }
+bool uses_mfma(const phylo_ctx* c) { return c->S > 4; }
 
 // Appends the full-evaluation program of a tree: internal nodes in the
 // reference's postorder (left subtree, right subtree, node), carry A holding
@@ -841,11 +847,11 @@ int upload_pmatrices(phylo_ctx* c) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     if (Sp > 4) {
-        const long n_mat = (long)c->n_nodes * c->C;
-        const long nPT = n_mat * (c->S_model + 1) * Sp;
-        HIPCHK(c, ensure(c->d_PT, c->PT_bytes, (size_t)nPT * sizeof(double)));
-        hipLaunchKernelGGL(ptranspose_kernel, dim3((unsigned)((nPT + 255) / 256)), dim3(256), 0, c->stream, c->d_P,
-                           c->S_model, Sp, n_mat, c->d_PT);
+        const int V = mfma_slots(c->S_model), TN = V > 4 ? 2 : 1;
+        const long nPm = (long)c->n_nodes * c->C * TN * V * 64;
+        HIPCHK(c, ensure(c->d_Pm, c->Pm_bytes, (size_t)nPm * sizeof(double)));
+        hipLaunchKernelGGL(pmfma_kernel, dim3((unsigned)((nPm + 255) / 256)), dim3(256), 0, c->stream, c->d_P,
+                           c->S_model, Sp, c->C, c->n_nodes, V, TN, c->d_Pm);
         HIPCHK(c, hipGetLastError());
     }
     c->dirty_P = false;
@@ -901,7 +907,7 @@ int prepare(phylo_ctx* c) {
     }
     const size_t elt = c->prec == PHYLO_PREC_FP64 ? sizeof(double) : sizeof(float);
     HIPCHK(c, ensure(c->d_clv, c->clv_bytes,
-                     (size_t)std::max(c->n_slots, 1) * c->C * c->m_pad * padded_states(c->S) * elt));
+                     (size_t)std::max(c->n_slots, 1) * c->C * c->m_pad * clv_states(c) * elt));
     HIPCHK(c, ensure(c->d_scl, c->scl_bytes, (size_t)std::max(c->n_slots, 1) * c->m_pad * sizeof(int)));
     if (c->dirty_P) {
         int rc = upload_pmatrices(c);
@@ -920,38 +926,6 @@ int prepare(phylo_ctx* c) {
         c->dirty_roots = false;
     }
     return PHYLO_OK;
-}
-
-template <typename T, int C, int SP>
-void launch_wide(phylo_ctx* c, dim3 grid, double* site_lnl) {
-    const uint32_t full = c->S == 32 ? 0xFFFFFFFFu : ((1u << c->S) - 1u);
-    hipLaunchKernelGGL((prune_wide_kernel<T, C, SP>), grid, dim3(64 * C), 0, c->stream, c->run_ops, c->run_prog,
-                       reinterpret_cast<const uint32_t*>(c->d_tips), reinterpret_cast<T*>(c->d_clv), c->d_scl, c->d_P,
-                       c->d_PT, c->d_roots, c->have_site_w ? c->d_sitew : nullptr, site_lnl, c->d_partial, c->m,
-                       c->m_pad, c->S, full);
-}
-
-template <typename T, int C>
-int dispatch_wide_sp(phylo_ctx* c, dim3 grid, double* site_lnl) {
-    switch (padded_states(c->S)) {
-    case 8: launch_wide<T, C, 8>(c, grid, site_lnl); break;
-    case 16: launch_wide<T, C, 16>(c, grid, site_lnl); break;
-    case 20: launch_wide<T, C, 20>(c, grid, site_lnl); break;
-    case 24: launch_wide<T, C, 24>(c, grid, site_lnl); break;
-    default: launch_wide<T, C, 32>(c, grid, site_lnl); break;
-    }
-    return PHYLO_OK;
-}
-
-template <typename T>
-int dispatch_wide(phylo_ctx* c, dim3 grid, double* site_lnl) {
-    switch (c->C) {
-    case 1: return dispatch_wide_sp<T, 1>(c, grid, site_lnl);
-    case 2: return dispatch_wide_sp<T, 2>(c, grid, site_lnl);
-    case 4: return dispatch_wide_sp<T, 4>(c, grid, site_lnl);
-    case 8: return dispatch_wide_sp<T, 8>(c, grid, site_lnl);
-    default: return fail(c, PHYLO_ERR_UNSUPPORTED, "number of rate categories must be 1, 2, 4 or 8");
-    }
 }
 
 template <typename T, int C>
@@ -974,13 +948,47 @@ int dispatch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
     return PHYLO_OK;
 }
 
+template <typename T, int C, int V>
+void launch_mfma(phylo_ctx* c, dim3 grid, double* site_lnl) {
+    hipLaunchKernelGGL((prune_mfma_kernel<T, C, V>), grid, dim3(64 * C), 0, c->stream, c->run_ops, c->run_prog,
+                       reinterpret_cast<const uint32_t*>(c->d_tips), reinterpret_cast<T*>(c->d_clv), c->d_scl,
+                       c->d_Pm, c->d_roots, c->have_site_w ? c->d_sitew : nullptr, site_lnl, c->d_partial, c->m,
+                       c->m_pad);
+}
+
+template <typename T, int C>
+int dispatch_mfma_v(phylo_ctx* c, dim3 grid, double* site_lnl) {
+    switch (mfma_slots(c->S)) {
+    case 2: launch_mfma<T, C, 2>(c, grid, site_lnl); break;
+    case 3: launch_mfma<T, C, 3>(c, grid, site_lnl); break;
+    case 4: launch_mfma<T, C, 4>(c, grid, site_lnl); break;
+    case 5: launch_mfma<T, C, 5>(c, grid, site_lnl); break;
+    case 6: launch_mfma<T, C, 6>(c, grid, site_lnl); break;
+    case 7: launch_mfma<T, C, 7>(c, grid, site_lnl); break;
+    default: launch_mfma<T, C, 8>(c, grid, site_lnl); break;
+    }
+    return PHYLO_OK;
+}
+
+template <typename T>
+int dispatch_mfma(phylo_ctx* c, dim3 grid, double* site_lnl) {
+    switch (c->C) {
+    case 1: return dispatch_mfma_v<T, 1>(c, grid, site_lnl);
+    case 2: return dispatch_mfma_v<T, 2>(c, grid, site_lnl);
+    case 4: return dispatch_mfma_v<T, 4>(c, grid, site_lnl);
+    case 8: return dispatch_mfma_v<T, 8>(c, grid, site_lnl);
+    default: return fail(c, PHYLO_ERR_UNSUPPORTED, "number of rate categories must be 1, 2, 4 or 8");
+    }
+}
+
 // Launches the program set currently in c->ops / c->prog_begin and returns
 // one total per program in totals (n_prog values); site_lnl [n_prog][m] is optional.
 int run_programs(phylo_ctx* c, const Op* d_ops, const int* d_prog, int n_prog, double* totals,
                  double* site_lnl_host) {
     c->run_ops = d_ops;
     c->run_prog = d_prog;
-    const int nblk = c->m_pad / 64;
+    const bool mfma = uses_mfma(c);
+    const int nblk = c->m_pad / (mfma ? 16 : 64);
     HIPCHK(c, ensure(c->d_partial, c->partial_bytes, (size_t)n_prog * nblk * sizeof(double)));
     HIPCHK(c, ensure(c->d_total, c->total_bytes, (size_t)n_prog * sizeof(double)));
     if (site_lnl_host) HIPCHK(c, ensure(c->d_sitelnl, c->sitelnl_bytes, (size_t)n_prog * c->m * sizeof(double)));
@@ -997,7 +1005,7 @@ int run_programs(phylo_ctx* c, const Op* d_ops, const int* d_prog, int n_prog, d
     if (c->S <= 4)
         rc = c->prec == PHYLO_PREC_FP64 ? dispatch_prune4<double>(c, grid, dsite) : dispatch_prune4<float>(c, grid, dsite);
     else
-        rc = c->prec == PHYLO_PREC_FP64 ? dispatch_wide<double>(c, grid, dsite) : dispatch_wide<float>(c, grid, dsite);
+        rc = c->prec == PHYLO_PREC_FP64 ? dispatch_mfma<double>(c, grid, dsite) : dispatch_mfma<float>(c, grid, dsite);
     if (rc) return rc;
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -1176,7 +1184,7 @@ void phylo_destroy(phylo_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void* bufs[] = {c->d_tips, c->d_PT, c->d_clv, c->d_scl, c->d_P, c->d_ops, c->d_prog, c->d_nni_ops, c->d_nni_prog,
+    void* bufs[] = {c->d_tips, c->d_Pm, c->d_clv, c->d_scl, c->d_P, c->d_ops, c->d_prog, c->d_nni_ops, c->d_nni_prog,
                     c->d_roots, c->d_sitew,
                     c->d_sitelnl, c->d_partial, c->d_total, c->d_eig, c->d_eff};
     for (void* b : bufs)
@@ -1347,11 +1355,30 @@ int phylo_node_partials(phylo_ctx* c, int node, double* clv, int32_t* scale) {
     if (!c->d_clv || c->ops.empty()) return fail(c, PHYLO_ERR_STATE, "node_partials before loglik");
     if (node < c->n_taxa || node >= c->n_nodes) return fail(c, PHYLO_ERR_ARG, "node_partials: not an internal node");
     const int slot = node - c->n_taxa;
-    const int Sp = padded_states(c->S);
+    const int Sp = clv_states(c);
     const size_t n = (size_t)c->C * c->m_pad * Sp;
     std::vector<int> sc(c->m_pad);
     HIPCHK(c, hipSetDevice(c->device));
-    if (c->prec == PHYLO_PREC_FP64) {
+    if (uses_mfma(c)) {
+        // [cat][site group][V][64 lanes]: state i of site s is slot i / 4 of lane (i % 4) * 16 + s % 16
+        const int V = mfma_slots(c->S), n_sg = c->m_pad / 16;
+        std::vector<double> buf(n);
+        if (c->prec == PHYLO_PREC_FP64) {
+            HIPCHK(c, hipMemcpyAsync(buf.data(), reinterpret_cast<double*>(c->d_clv) + slot * n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        } else {
+            std::vector<float> fbuf(n);
+            HIPCHK(c, hipMemcpyAsync(fbuf.data(), reinterpret_cast<float*>(c->d_clv) + slot * n, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            std::copy(fbuf.begin(), fbuf.end(), buf.begin());
+        }
+        HIPCHK(c, hipMemcpyAsync(sc.data(), c->d_scl + (size_t)slot * c->m_pad, c->m_pad * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int s = 0; s < c->m; ++s)
+            for (int k = 0; k < c->C; ++k)
+                for (int i = 0; i < c->S; ++i)
+                    clv[((size_t)s * c->C + k) * c->S + i] =
+                        buf[(((size_t)k * n_sg + s / 16) * V + i / 4) * 64 + (i % 4) * 16 + s % 16];
+    } else if (c->prec == PHYLO_PREC_FP64) {
         std::vector<double> buf(n);
         HIPCHK(c, hipMemcpyAsync(buf.data(), reinterpret_cast<double*>(c->d_clv) + slot * n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipMemcpyAsync(sc.data(), c->d_scl + (size_t)slot * c->m_pad, c->m_pad * sizeof(int), hipMemcpyDeviceToHost, c->stream));

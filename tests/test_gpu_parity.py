@@ -65,3 +65,25 @@ def test_wide_state_loglik_matches_oracle(oracle, n, m, S, C, alpha, gap, seed, 
     else:
         assert rel.max() <= 1e-5, rel.max()
         assert abs(tot - ref_tot) <= 1e-3, abs(tot - ref_tot)
+
+
+@pytest.mark.parametrize("n,m,S,C,prec", [(12, 300, 4, 4, 0), (12, 300, 4, 2, 1), (10, 333, 20, 4, 0),
+                                          (9, 200, 20, 4, 1), (8, 150, 7, 2, 0), (8, 150, 29, 1, 0)])
+def test_node_partials_match_oracle(oracle, n, m, S, C, prec):
+    """phylo_node_partials (every storage layout: DNA float4 rows, wide FP64
+    rows, the MFMA walker's operand order) against the oracle's partials,
+    compared as value x 2^scaler."""
+    print(f"node partials n={n} m={m} S={S} C={C} prec={prec}", flush=True)
+    kw = dict(rate_dist="gamma", freq_conc=2.0) if S > 4 else {}
+    wl = synth.gtr_gamma_workload(n_tips=n, m=m, S=S, C=C, alpha=0.8, gap_frac=0.05, seed=S + n, **kw)
+    clv_o, sc_o, _, _ = oracle_problem(oracle, wl).node_clvs()
+    t = wl.tree
+    with gpu_context(wl, precision=prec) as ctx:
+        ctx.loglik()
+        for v in range(t.n_tips, t.n_nodes):
+            clv, sc = ctx.node_partials(v, C, S)
+            got = clv * np.ldexp(1.0, sc)[:, None, None]
+            want = clv_o[v] * np.ldexp(1.0, sc_o[v])[:, None, None]
+            scale = np.abs(want).max(axis=(1, 2), keepdims=True)
+            tol = 1e-10 if prec == 1 else 2e-5
+            np.testing.assert_allclose(got / scale, want / scale, atol=tol, err_msg=f"node {v}")
