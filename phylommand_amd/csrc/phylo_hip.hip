@@ -358,140 +358,244 @@ __global__ void pmfma_kernel(const double* __restrict__ P, int S, int Sp, int C,
     Pm[gid] = (row < S && col < S) ? P[(mat * Sp + row) * Sp + col] : 0.0;
 }
 
+// PTm[node][cat][k][i] = P[i][k] for k < S, PTm[..][S][i] = sum_q P[i][q]
+// (the product with an all-states tip), i < 4V (0 for i >= S): the columns a
+// single-state or all-states tip selects.
+__global__ void ptm_kernel(const double* __restrict__ P, int S, int Sp, int C, int n_nodes, int W,
+                           double* __restrict__ PTm) {
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)n_nodes * C * (S + 1) * W;
+    if (gid >= total) return;
+    const int i = gid % W;
+    const int k = (gid / W) % (S + 1);
+    const long mat = gid / ((long)W * (S + 1));
+    const double* row = P + (mat * Sp + i) * Sp;
+    double v = 0.0;
+    if (i < S) {
+        if (k < S) v = row[k];
+        else
+            for (int q = 0; q < S; ++q) v += row[q];
+    }
+    PTm[gid] = v;
+}
+
 __device__ inline double to_d(float x) { return (double)x; }
 __device__ inline double to_d(double x) { return x; }
 
-// One child: acc[T] = P . x for tile T, x in operand order.
-template <typename T, int C, int V>
-__device__ inline void mfma_child(int kind, int idx, int node, int cat, int s, int sg, int n_sg, int lane, int g,
-                                  int m_pad, const uint32_t* __restrict__ tips, const T* clv, size_t slot_stride,
-                                  const int* scl, const double* __restrict__ Pm, const double (&cA)[V], int sA,
-                                  const double (&cB)[V], int sB, mfma_d4 (&acc)[2], int& sc) {
+// One child for the G site groups of a wave: acc[q][T] = P . x for tile T,
+// x in operand order.  The P operand of each K-step is loaded once and used by
+// every group.
+template <typename T, int C, int V, int G>
+__device__ inline void mfma_child(int kind, int idx, int node, int cat, const int (&s)[G], int sg0, int n_sg,
+                                  int lane, int g, int m_pad, int S, uint32_t full,
+                                  const uint32_t* __restrict__ tips, const T* clv, size_t slot_stride,
+                                  const int* scl, const double* __restrict__ Pm, const double* __restrict__ PTm,
+                                  const T (&cA)[G][V], const int (&sA)[G], const T (&cB)[G][V], const int (&sB)[G],
+                                  mfma_d4 (&acc)[G][2], int (&sc)[G]) {
     constexpr int TN = V > 4 ? 2 : 1;
-    double x[V];
+    double x[G][V];
     if (kind == SRC_TIP) {
-        const uint32_t mk = tips[(size_t)idx * m_pad + s];
+        uint32_t mk[G];
+        bool simple = true;
 #pragma unroll
-        for (int t = 0; t < V; ++t) x[t] = (mk >> (4 * t + g)) & 1u ? 1.0 : 0.0;
+        for (int q = 0; q < G; ++q) {
+            mk[q] = tips[(size_t)idx * m_pad + s[q]];
+            simple = simple && (mk[q] == full || __popc(mk[q]) == 1);
+        }
+        // a wave whose sites are all single states or gaps selects columns of P
+        if (__all(simple)) {
+            const double* base = PTm + ((size_t)node * C + cat) * (S + 1) * (4 * V) + g;
+#pragma unroll
+            for (int q = 0; q < G; ++q) {
+                const int col = mk[q] == full ? S : __ffs(mk[q]) - 1;
+                const double* pt = base + (size_t)col * (4 * V);
+#pragma unroll
+                for (int t = 0; t < V; ++t) acc[q][t / 4][t % 4] = pt[4 * t];
+            }
+            return;
+        }
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+#pragma unroll
+            for (int t = 0; t < V; ++t) x[q][t] = (mk[q] >> (4 * t + g)) & 1u ? 1.0 : 0.0;
     } else if (kind == SRC_CLV) {
-        const T* src = clv + (size_t)idx * slot_stride + ((size_t)cat * n_sg + sg) * V * 64 + lane;
 #pragma unroll
-        for (int t = 0; t < V; ++t) x[t] = to_d(src[t * 64]);
-        sc += scl[(size_t)idx * m_pad + s];
+        for (int q = 0; q < G; ++q) {
+            const T* src = clv + (size_t)idx * slot_stride + ((size_t)cat * n_sg + sg0 + q) * V * 64 + lane;
+#pragma unroll
+            for (int t = 0; t < V; ++t) x[q][t] = to_d(src[t * 64]);
+            sc[q] += scl[(size_t)idx * m_pad + s[q]];
+        }
     } else if (kind == SRC_CARRY_A) {
 #pragma unroll
-        for (int t = 0; t < V; ++t) x[t] = cA[t];
-        sc += sA;
+        for (int q = 0; q < G; ++q) {
+#pragma unroll
+            for (int t = 0; t < V; ++t) x[q][t] = to_d(cA[q][t]);
+            sc[q] += sA[q];
+        }
     } else {
 #pragma unroll
-        for (int t = 0; t < V; ++t) x[t] = cB[t];
-        sc += sB;
+        for (int q = 0; q < G; ++q) {
+#pragma unroll
+            for (int t = 0; t < V; ++t) x[q][t] = to_d(cB[q][t]);
+            sc[q] += sB[q];
+        }
     }
     const double* pm = Pm + ((size_t)node * C + cat) * TN * V * 64 + lane;
 #pragma unroll
     for (int T2 = 0; T2 < TN; ++T2) {
-        acc[T2] = mfma_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int t = 0; t < V; ++t)
-            acc[T2] = __builtin_amdgcn_mfma_f64_16x16x4f64(pm[(T2 * V + t) * 64], x[t], acc[T2], 0, 0, 0);
+        for (int q = 0; q < G; ++q) acc[q][T2] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < V; ++t) {
+            const double a = pm[(T2 * V + t) * 64];
+#pragma unroll
+            for (int q = 0; q < G; ++q)
+                acc[q][T2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, x[q][t], acc[q][T2], 0, 0, 0);
+        }
     }
 }
 
-// grid = (m_pad / 16, n_programs), block = 64 * C.
-template <typename T, int C, int V>
+// grid = (m_pad / (16 G), n_programs), block = 64 * C: wave c computes
+// category c of the G 16-site groups starting at group G * blockIdx.x.
+template <typename T, int C, int V, int G>
 __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
     const Op* __restrict__ ops, const int* __restrict__ prog_begin, const uint32_t* __restrict__ tips, T* clv,
-    int* scl, const double* __restrict__ Pm, const double* __restrict__ roots, const double* __restrict__ site_w,
-    double* __restrict__ site_lnl, double* __restrict__ partial, int m, int m_pad) {
-    constexpr int TN = V > 4 ? 2 : 1;
-    __shared__ int xexp[2][C][16];
-    __shared__ double xroot[C][16];
+    int* scl, const double* __restrict__ Pm, const double* __restrict__ PTm, const double* __restrict__ roots,
+    const double* __restrict__ site_w, double* __restrict__ site_lnl, double* __restrict__ partial, int m, int m_pad,
+    int S, uint32_t full) {
+    __shared__ int xexp[2][C][16 * G];
+    __shared__ double xroot[C][16 * G];
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, j = lane & 15;
     const int cat = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int sg = blockIdx.x, n_sg = m_pad >> 4;
-    const int s = sg * 16 + j;
+    const int sg0 = blockIdx.x * G, n_sg = m_pad >> 4;
+    int s[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) s[q] = (sg0 + q) * 16 + j;
     const int prog = blockIdx.y;
     const int k0 = prog_begin[prog], k1 = prog_begin[prog + 1];
     const size_t slot_stride = (size_t)C * m_pad * 4 * V;
-    double cA[V], cB[V];
+    T cA[G][V], cB[G][V];
+    int sA[G], sB[G];
 #pragma unroll
-    for (int t = 0; t < V; ++t) { cA[t] = 0.0; cB[t] = 0.0; }
-    int sA = 0, sB = 0;
-    double lnl = 0.0;
+    for (int q = 0; q < G; ++q) {
+#pragma unroll
+        for (int t = 0; t < V; ++t) { cA[q][t] = T(0); cB[q][t] = T(0); }
+        sA[q] = 0;
+        sB[q] = 0;
+    }
+    double lnl[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) lnl[q] = 0.0;
     for (int k = k0; k < k1; ++k) {
         const Op op = ops[k];
-        int sc = 0;
-        mfma_d4 la[2], ra[2];
-        mfma_child<T, C, V>(op.l_kind, op.l_idx, op.l_node, cat, s, sg, n_sg, lane, g, m_pad, tips, clv, slot_stride,
-                            scl, Pm, cA, sA, cB, sB, la, sc);
-        double val[V];
+        int sc[G];
+#pragma unroll
+        for (int q = 0; q < G; ++q) sc[q] = 0;
+        mfma_d4 acc[G][2];
+        double val[G][V];
+        mfma_child<T, C, V, G>(op.l_kind, op.l_idx, op.l_node, cat, s, sg0, n_sg, lane, g, m_pad, S, full, tips, clv,
+                               slot_stride, scl, Pm, PTm, cA, sA, cB, sB, acc, sc);
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+#pragma unroll
+            for (int t = 0; t < V; ++t) val[q][t] = acc[q][t / 4][t % 4];
         if (op.r_kind != SRC_NONE) {
-            mfma_child<T, C, V>(op.r_kind, op.r_idx, op.r_node, cat, s, sg, n_sg, lane, g, m_pad, tips, clv,
-                                slot_stride, scl, Pm, cA, sA, cB, sB, ra, sc);
+            mfma_child<T, C, V, G>(op.r_kind, op.r_idx, op.r_node, cat, s, sg0, n_sg, lane, g, m_pad, S, full, tips,
+                                   clv, slot_stride, scl, Pm, PTm, cA, sA, cB, sB, acc, sc);
 #pragma unroll
-            for (int t = 0; t < V; ++t) val[t] = la[t / 4][t % 4] * ra[t / 4][t % 4];
-        } else {
+            for (int q = 0; q < G; ++q)
 #pragma unroll
-            for (int t = 0; t < V; ++t) val[t] = la[t / 4][t % 4];
+                for (int t = 0; t < V; ++t) val[q][t] *= acc[q][t / 4][t % 4];
         }
-        (void)TN;
         // per-site maximum over the lane groups, then over the category waves
-        double mx = 0.0;
+        int e[G];
 #pragma unroll
-        for (int t = 0; t < V; ++t) mx = fmax(mx, val[t]);
-        mx = fmax(mx, __shfl_xor(mx, 16, 64));
-        mx = fmax(mx, __shfl_xor(mx, 32, 64));
-        int e = INT_MIN;
-        if (mx > 0.0) (void)frexp(mx, &e);
+        for (int q = 0; q < G; ++q) {
+            double mx = 0.0;
+#pragma unroll
+            for (int t = 0; t < V; ++t) mx = fmax(mx, val[q][t]);
+            mx = fmax(mx, __shfl_xor(mx, 16, 64));
+            mx = fmax(mx, __shfl_xor(mx, 32, 64));
+            e[q] = INT_MIN;
+            if (mx > 0.0) (void)frexp(mx, &e[q]);
+        }
         if (C > 1) {
             const int buf = k & 1;
-            if (g == 0) xexp[buf][cat][j] = e;
+            if (g == 0) {
+#pragma unroll
+                for (int q = 0; q < G; ++q) xexp[buf][cat][16 * q + j] = e[q];
+            }
             __syncthreads();
 #pragma unroll
-            for (int c = 0; c < C; ++c) e = max(e, xexp[buf][c][j]);
+            for (int q = 0; q < G; ++q)
+#pragma unroll
+                for (int c = 0; c < C; ++c) e[q] = max(e[q], xexp[buf][c][16 * q + j]);
         }
-        if (e == INT_MIN) e = 0;
-        sc += e;
 #pragma unroll
-        for (int t = 0; t < V; ++t) val[t] = to_d((T)ldexp(val[t], -e));   // stored precision, carried too
-        if (op.out_slot >= 0) {
-            T* dst = clv + (size_t)op.out_slot * slot_stride + ((size_t)cat * n_sg + sg) * V * 64 + lane;
+        for (int q = 0; q < G; ++q) {
+            if (e[q] == INT_MIN) e[q] = 0;
+            sc[q] += e[q];
+            T r[V];                                  // stored and carried in the storage precision
 #pragma unroll
-            for (int t = 0; t < V; ++t) dst[t * 64] = (T)val[t];
-            if (cat == 0 && g == 0) scl[(size_t)op.out_slot * m_pad + s] = sc;
-        }
-        if (op.flags & OP_TO_B) {
+            for (int t = 0; t < V; ++t) {
+                val[q][t] = ldexp(val[q][t], -e[q]);
+                r[t] = (T)val[q][t];
+            }
+            if (op.out_slot >= 0) {
+                T* dst = clv + (size_t)op.out_slot * slot_stride + ((size_t)cat * n_sg + sg0 + q) * V * 64 + lane;
 #pragma unroll
-            for (int t = 0; t < V; ++t) cB[t] = val[t];
-            sB = sc;
-        } else {
+                for (int t = 0; t < V; ++t) dst[t * 64] = r[t];
+                if (cat == 0 && g == 0) scl[(size_t)op.out_slot * m_pad + s[q]] = sc[q];
+            }
+            if (op.flags & OP_TO_B) {
 #pragma unroll
-            for (int t = 0; t < V; ++t) cA[t] = val[t];
-            sA = sc;
+                for (int t = 0; t < V; ++t) cB[q][t] = r[t];
+                sB[q] = sc[q];
+            } else {
+#pragma unroll
+                for (int t = 0; t < V; ++t) cA[q][t] = r[t];
+                sA[q] = sc[q];
+            }
         }
         if (op.flags & OP_ROOT) {
-            double lc = 0.0;
+            double lc[G];
 #pragma unroll
-            for (int t = 0; t < V; ++t) lc = fma(roots[4 * t + g], val[t], lc);
-            lc += __shfl_xor(lc, 16, 64);
-            lc += __shfl_xor(lc, 32, 64);
-            lc *= roots[kRootCatOff + cat];
+            for (int q = 0; q < G; ++q) {
+                lc[q] = 0.0;
+#pragma unroll
+                for (int t = 0; t < V; ++t) lc[q] = fma(roots[4 * t + g], val[q][t], lc[q]);
+                lc[q] += __shfl_xor(lc[q], 16, 64);
+                lc[q] += __shfl_xor(lc[q], 32, 64);
+                lc[q] *= roots[kRootCatOff + cat];
+            }
             if (C > 1) {
-                if (g == 0) xroot[cat][j] = lc;
-                __syncthreads();
-                lc = 0.0;
+                if (g == 0) {
 #pragma unroll
-                for (int c = 0; c < C; ++c) lc += xroot[c][j];
+                    for (int q = 0; q < G; ++q) xroot[cat][16 * q + j] = lc[q];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < G; ++q) {
+                    lc[q] = 0.0;
+#pragma unroll
+                    for (int c = 0; c < C; ++c) lc[q] += xroot[c][16 * q + j];
+                }
                 __syncthreads();
             }
-            lnl = log(lc) + sc * kLn2;
+#pragma unroll
+            for (int q = 0; q < G; ++q) lnl[q] = log(lc[q]) + sc[q] * kLn2;
         }
     }
-    const bool live = s < m && cat == 0 && g == 0;
-    if (live && site_lnl) site_lnl[(size_t)prog * m + s] = lnl;
-    const double w = live ? (site_w ? site_w[s] : 1.0) : 0.0;
-    block_sum_store(live ? w * lnl : 0.0, partial + (size_t)prog * gridDim.x + blockIdx.x);
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+        const bool live = s[q] < m && cat == 0 && g == 0;
+        if (live && site_lnl) site_lnl[(size_t)prog * m + s[q]] = lnl[q];
+        if (live) sum += (site_w ? site_w[s[q]] : 1.0) * lnl[q];
+    }
+    block_sum_store(sum, partial + (size_t)prog * gridDim.x + blockIdx.x);
 }
 
 // Compensated fixed-order sum of the per-workgroup partials of each program
@@ -641,6 +745,7 @@ struct phylo_ctx {
     // device buffers
     void* d_tips = nullptr; size_t tips_bytes = 0;     // uint8 codes (S <= 4) or uint32 masks
     double* d_Pm = nullptr; size_t Pm_bytes = 0;       // P in MFMA operand order (wide states)
+    double* d_PTm = nullptr; size_t PTm_bytes = 0;     // P columns for tips (wide states)
     void* d_clv = nullptr; size_t clv_bytes = 0;
     int* d_scl = nullptr; size_t scl_bytes = 0;
     double* d_P = nullptr; size_t P_bytes = 0;
@@ -853,6 +958,11 @@ int upload_pmatrices(phylo_ctx* c) {
         hipLaunchKernelGGL(pmfma_kernel, dim3((unsigned)((nPm + 255) / 256)), dim3(256), 0, c->stream, c->d_P,
                            c->S_model, Sp, c->C, c->n_nodes, V, TN, c->d_Pm);
         HIPCHK(c, hipGetLastError());
+        const long nPT = (long)c->n_nodes * c->C * (c->S_model + 1) * 4 * V;
+        HIPCHK(c, ensure(c->d_PTm, c->PTm_bytes, (size_t)nPT * sizeof(double)));
+        hipLaunchKernelGGL(ptm_kernel, dim3((unsigned)((nPT + 255) / 256)), dim3(256), 0, c->stream, c->d_P,
+                           c->S_model, Sp, c->C, c->n_nodes, 4 * V, c->d_PTm);
+        HIPCHK(c, hipGetLastError());
     }
     c->dirty_P = false;
     return PHYLO_OK;
@@ -948,12 +1058,17 @@ int dispatch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
     return PHYLO_OK;
 }
 
+// Site groups per wave of the MFMA walker: two independent MFMA chains per
+// wave share each P operand load and overlap their memory latency.
+constexpr int kMfmaGroups = 2;
+
 template <typename T, int C, int V>
 void launch_mfma(phylo_ctx* c, dim3 grid, double* site_lnl) {
-    hipLaunchKernelGGL((prune_mfma_kernel<T, C, V>), grid, dim3(64 * C), 0, c->stream, c->run_ops, c->run_prog,
+    hipLaunchKernelGGL((prune_mfma_kernel<T, C, V, kMfmaGroups>), grid, dim3(64 * C), 0, c->stream, c->run_ops,
+                       c->run_prog,
                        reinterpret_cast<const uint32_t*>(c->d_tips), reinterpret_cast<T*>(c->d_clv), c->d_scl,
-                       c->d_Pm, c->d_roots, c->have_site_w ? c->d_sitew : nullptr, site_lnl, c->d_partial, c->m,
-                       c->m_pad);
+                       c->d_Pm, c->d_PTm, c->d_roots, c->have_site_w ? c->d_sitew : nullptr, site_lnl, c->d_partial,
+                       c->m, c->m_pad, c->S, c->S == 32 ? 0xFFFFFFFFu : ((1u << c->S) - 1u));
 }
 
 template <typename T, int C>
@@ -988,7 +1103,7 @@ int run_programs(phylo_ctx* c, const Op* d_ops, const int* d_prog, int n_prog, d
     c->run_ops = d_ops;
     c->run_prog = d_prog;
     const bool mfma = uses_mfma(c);
-    const int nblk = c->m_pad / (mfma ? 16 : 64);
+    const int nblk = c->m_pad / (mfma ? 16 * kMfmaGroups : 64);
     HIPCHK(c, ensure(c->d_partial, c->partial_bytes, (size_t)n_prog * nblk * sizeof(double)));
     HIPCHK(c, ensure(c->d_total, c->total_bytes, (size_t)n_prog * sizeof(double)));
     if (site_lnl_host) HIPCHK(c, ensure(c->d_sitelnl, c->sitelnl_bytes, (size_t)n_prog * c->m * sizeof(double)));
@@ -1184,7 +1299,7 @@ void phylo_destroy(phylo_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void* bufs[] = {c->d_tips, c->d_Pm, c->d_clv, c->d_scl, c->d_P, c->d_ops, c->d_prog, c->d_nni_ops, c->d_nni_prog,
+    void* bufs[] = {c->d_tips, c->d_Pm, c->d_PTm, c->d_clv, c->d_scl, c->d_P, c->d_ops, c->d_prog, c->d_nni_ops, c->d_nni_prog,
                     c->d_roots, c->d_sitew,
                     c->d_sitelnl, c->d_partial, c->d_total, c->d_eig, c->d_eff};
     for (void* b : bufs)
