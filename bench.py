@@ -223,6 +223,52 @@ def bench_batch(args, P, synth, n_inst=512):
                                             "finite": bool(np.all(np.isfinite(tot)) and np.all(np.isfinite(nni)))}}
 
 
+def bench_learn(args, P):
+    """Learned proposal: simulate and featurise --train alignments (32 taxa x
+    2000 sites, random GTR+G4 on Yule trees) on the GPU, train the
+    256-512-512-12 MLP on the GPU, then fit --heldout new alignments twice on
+    their true topologies (FP64 partials): Nelder-Mead from the default start
+    (equal rates and frequencies, alpha 1, mean length 0.1, simplex edge 0.3)
+    and from the MLP's prediction (simplex edges twice its held-out RMSE);
+    both stop at an lnL spread of 1e-3 and restart with a 0.01 simplex until a
+    restart gains less than that."""
+    from phylommand_amd import learn as L
+    rng = np.random.default_rng(5)
+    t0 = time.perf_counter()
+    draws = L.draw_models(args.train, 32, rng)
+    feats = L.simulate_features(draws, 2000, seed=11)
+    t_sim = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    prop = L.Proposal()
+    losses = prop.fit(feats, L.targets(draws), schedule=[(20, 1e-3), (10, 3e-4), (10, 1e-4)])
+    t_train = time.perf_counter() - t0
+    print(f"[bench] learn: simulated {args.train} in {t_sim:.1f}s, trained in {t_train:.1f}s", file=sys.stderr,
+          flush=True)
+    held = L.draw_models(args.heldout, 32, np.random.default_rng(99))
+    hf, hm = L.simulate_features(held, 2000, seed=77, want_masks=True)
+    common = dict(ftol_abs=1e-3, xtol_rel=1e-4, max_evals=8000, restart_step=0.01)
+    c = L.compare_fits(prop, held, hf, hm, opts=P.nm_options(**common),
+                       seeded_opts=P.nm_options(**common, steps=2 * prop.x_rmse))
+    diff = c.lnl_seeded - c.lnl_plain
+    ratio = c.sec_plain.sum() / c.sec_seeded.sum()
+    return {"metric": "ML-fit wall time, plain / learned-seeded (matched lnL)", "value": ratio, "unit": "x",
+            "n_gpus": 1, "steps": args.heldout, "warmup": 0,
+            "ms_per_step": 1e3 * c.sec_seeded.sum() / args.heldout, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic (GPU-simulated GTR+G4 alignments)",
+            "config": {"workload": f"learned proposal: {args.train} training alignments, {args.heldout} held-out "
+                                   f"fits, 32 taxa x 2000 sites, GTR+G4",
+                       "plain_s": c.sec_plain.sum(), "seeded_s": c.sec_seeded.sum(),
+                       "plain_evals_mean": float(c.evals_plain.mean()),
+                       "seeded_evals_mean": float(c.evals_seeded.mean()),
+                       "lnl_within_0.01": int(np.sum(np.abs(diff) <= 0.01)),
+                       "seeded_higher_by_more": int(np.sum(diff > 0.01)),
+                       "plain_higher_by_more": int(np.sum(diff < -0.01)),
+                       "max_plain_minus_seeded": float(max(0.0, -diff.min())),
+                       "max_seeded_minus_plain": float(max(0.0, diff.max())),
+                       "final_train_loss": float(losses[-1]), "simulate_s": t_sim, "train_s": t_train,
+                       "proposal_rmse": [float(v) for v in prop.x_rmse]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -232,7 +278,9 @@ def main():
     ap.add_argument("--taxa", type=int, default=128)
     ap.add_argument("--fp64", action="store_true", help="store partials in FP64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="dna", choices=["dna", "protein", "nni", "batch"])
+    ap.add_argument("--workload", default="dna", choices=["dna", "protein", "nni", "batch", "learn"])
+    ap.add_argument("--train", type=int, default=200_000, help="learn: training alignments")
+    ap.add_argument("--heldout", type=int, default=1000, help="learn: held-out fits")
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run only a plain device-to-device copy of MIB MiB (torch), the calibration kernel "
@@ -258,6 +306,9 @@ def main():
     from phylommand_amd import synth
 
     if args.workload != "dna":
+        if args.workload == "learn":
+            print(json.dumps(bench_learn(args, P)))
+            return
         fn = {"protein": bench_protein, "nni": bench_nni}.get(args.workload)
         out = fn(args, P, synth) if fn else bench_batch(args, P, synth, args.instances)
         print(json.dumps(out))

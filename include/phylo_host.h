@@ -35,6 +35,8 @@ typedef struct phylo_nm_options {
     double initial_step;  /* simplex edge per coordinate */
     int adaptive;         /* 1: dimension-dependent coefficients (Gao & Han 2012) */
     int max_restarts;
+    const double *steps;  /* per-coordinate simplex edges (one per parameter), NULL: initial_step */
+    double restart_step;  /* largest simplex edge of a restart */
 } phylo_nm_options;
 int phylo_nm_defaults(phylo_nm_options *opt);
 double phylo_gtr_gamma_lnl(phylo_ctx *ctx, int n_nodes, const double *rel_len, int n_cat, const double *x);

@@ -120,15 +120,23 @@ _host_lib = None
 class NMOptions(ctypes.Structure):
     """phylo_nm_options (include/phylo_host.h)."""
     _fields_ = [("max_evals", ctypes.c_int), ("ftol_abs", ctypes.c_double), ("xtol_rel", ctypes.c_double),
-                ("initial_step", ctypes.c_double), ("adaptive", ctypes.c_int), ("max_restarts", ctypes.c_int)]
+                ("initial_step", ctypes.c_double), ("adaptive", ctypes.c_int), ("max_restarts", ctypes.c_int),
+                ("steps", ctypes.POINTER(ctypes.c_double)), ("restart_step", ctypes.c_double)]
 
 
 def nm_options(**kw) -> NMOptions:
-    """The library's Nelder-Mead defaults with the given fields replaced."""
+    """The library's Nelder-Mead defaults with the given fields replaced
+    (steps: one simplex edge per parameter)."""
     o = NMOptions()
     load_host_library().phylo_nm_defaults(ctypes.byref(o))
     for k, v in kw.items():
-        if v is not None:
+        if v is None:
+            continue
+        if k == "steps":
+            arr = np.ascontiguousarray(v, dtype=np.float64)
+            o._steps_keep = arr
+            o.steps = arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        else:
             setattr(o, k, v)
     return o
 

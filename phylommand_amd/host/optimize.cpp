@@ -32,7 +32,8 @@ nm_result nelder_mead(const std::function<double(const std::vector<double>&)>& f
     std::vector<std::vector<double>> s(n + 1, x0);
     std::vector<double> fs(n + 1);
     for (size_t i = 0; i < n; ++i) {
-        s[i + 1][i] += (x0[i] + opt.initial_step <= hi[i]) ? opt.initial_step : -opt.initial_step;
+        const double step = opt.steps.size() == n ? opt.steps[i] : opt.initial_step;
+        s[i + 1][i] += (x0[i] + step <= hi[i]) ? step : -step;
         clamp(s[i + 1]);
     }
     for (size_t i = 0; i <= n; ++i) fs[i] = eval(s[i]);
@@ -244,7 +245,8 @@ gtr_fit fit_gtr_gamma(phylo_ctx* ctx, int n_nodes, const double* rel_len, int n_
     for (int restart = 0; restart < opt.max_restarts && evals < opt.max_evals; ++restart) {
         nm_options o = opt;
         o.max_evals = opt.max_evals - evals;
-        o.initial_step = std::min(opt.initial_step, 0.1);
+        o.initial_step = std::min(opt.initial_step, opt.restart_step);
+        for (double& st : o.steps) st = std::min(st, opt.restart_step);
         const nm_result r2 = nelder_mead(f, r.x, lo, hi, o);
         evals += r2.evals;
         const bool gained = r2.f > r.f + opt.ftol_abs;
@@ -338,6 +340,8 @@ int phylo_nm_defaults(phylo_nm_options* o) {
     o->initial_step = d.initial_step;
     o->adaptive = d.adaptive;
     o->max_restarts = d.max_restarts;
+    o->steps = nullptr;
+    o->restart_step = d.restart_step;
     return PHYLO_OK;
 }
 
@@ -347,7 +351,7 @@ int phylo_fit_gtr_gamma(phylo_ctx* ctx, int n_nodes, const double* rel_len, int 
     phylo_amd::nm_options opt;
     if (nm) {
         if (nm->max_evals < 1 || !(nm->ftol_abs >= 0) || !(nm->xtol_rel >= 0) || !(nm->initial_step > 0) ||
-            nm->max_restarts < 0)
+            nm->max_restarts < 0 || !(nm->restart_step > 0))
             return PHYLO_ERR_ARG;
         opt.max_evals = nm->max_evals;
         opt.ftol_abs = nm->ftol_abs;
@@ -355,6 +359,12 @@ int phylo_fit_gtr_gamma(phylo_ctx* ctx, int n_nodes, const double* rel_len, int 
         opt.initial_step = nm->initial_step;
         opt.adaptive = nm->adaptive != 0;
         opt.max_restarts = nm->max_restarts;
+        opt.restart_step = nm->restart_step;
+        if (nm->steps) {
+            for (int i = 0; i < 10; ++i)
+                if (!(nm->steps[i] > 0)) return PHYLO_ERR_ARG;
+            opt.steps.assign(nm->steps, nm->steps + 10);
+        }
     }
     const phylo_amd::gtr_fit f =
         phylo_amd::fit_gtr_gamma(ctx, n_nodes, rel_len, n_cat, std::vector<double>(x0, x0 + 10), opt);

@@ -23,10 +23,12 @@ struct nm_options {
     double ftol_abs = 1e-6;      // stop when the simplex's lnL spread falls below this
     double xtol_rel = 1e-7;      // ... and its extent relative to the best point below sqrt(xtol_rel)
     double initial_step = 0.3;   // simplex edge (per coordinate, absolute)
+    std::vector<double> steps;   // per-coordinate edges; empty: initial_step for all
     // Dimension-dependent coefficients (Gao & Han 2012: expansion 1 + 2/n,
     // contraction 0.75 - 1/(2n), shrink 1 - 1/n) instead of 2 / 0.5 / 0.5.
     bool adaptive = false;
     int max_restarts = 4;        // fresh simplices around the optimum (model fits only)
+    double restart_step = 0.1;   // their largest edge
 };
 
 struct nm_result {

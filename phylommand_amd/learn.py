@@ -22,7 +22,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import MLP, Context, gamma_rates, nm_options, sim_features
+from . import MLP, PREC_FP64, Context, gamma_rates, nm_options, sim_features
 from .synth import yule_tree
 
 N_OUT = 12
@@ -140,19 +140,33 @@ def default_start(mean_blen: float = 0.1) -> np.ndarray:
 
 
 class Proposal:
-    """Feature / target standardisation plus the GPU MLP."""
+    """Feature / target standardisation plus the GPU MLP.  After fit(),
+    x_rmse holds the prediction's root-mean-square error per fit parameter on
+    a held-out part of the training draws: the proposal's own scale, used as
+    the seeded simplex's edges."""
 
     def __init__(self, n_hidden: int = 512, seed: int = 1, max_batch: int = 256):
         self.mlp = MLP(256, n_hidden, N_OUT, max_batch=max_batch, seed=seed)
         self.fmu = self.fsd = self.tmu = self.tsd = None
+        self.x_rmse = None
 
-    def fit(self, feats, targ, epochs=30, batch=256, lr=1e-3, seed=1):
-        self.fmu = feats.mean(axis=0)
-        self.fsd = np.maximum(feats.std(axis=0), 1e-6)
-        self.tmu = targ.mean(axis=0)
-        self.tsd = np.maximum(targ.std(axis=0), 1e-6)
-        return self.mlp.train((feats - self.fmu) / self.fsd, (targ - self.tmu) / self.tsd, epochs=epochs,
-                              batch=batch, lr=lr, seed=seed)
+    def fit(self, feats, targ, epochs=30, batch=256, lr=1e-3, seed=1, holdout=0.05, schedule=None):
+        """Trains on all but the last `holdout` fraction; schedule, a list of
+        (epochs, learning rate) stages, replaces (epochs, lr)."""
+        n_val = max(1, int(len(feats) * holdout))
+        tr_f, tr_t = feats[:-n_val], targ[:-n_val]
+        self.fmu = tr_f.mean(axis=0)
+        self.fsd = np.maximum(tr_f.std(axis=0), 1e-6)
+        self.tmu = tr_t.mean(axis=0)
+        self.tsd = np.maximum(tr_t.std(axis=0), 1e-6)
+        xs, ts = (tr_f - self.fmu) / self.fsd, (tr_t - self.tmu) / self.tsd
+        losses = []
+        for k, (ep, rate) in enumerate(schedule or [(epochs, lr)]):
+            losses.extend(self.mlp.train(xs, ts, epochs=ep, batch=batch, lr=rate, seed=seed + k))
+        losses = np.array(losses)
+        err = fit_start(self.predict(feats[-n_val:])) - fit_start(targ[-n_val:])
+        self.x_rmse = np.sqrt(np.mean(err ** 2, axis=0))
+        return losses
 
     def predict(self, feats) -> np.ndarray:
         y = self.mlp.forward((np.atleast_2d(feats) - self.fmu) / self.fsd)
@@ -169,7 +183,8 @@ class FitComparison:
     sec_seeded: np.ndarray
 
 
-def compare_fits(prop: Proposal, d: ModelDraws, feats, masks, n_cat=4, opts=None, seeded_opts=None) -> FitComparison:
+def compare_fits(prop: Proposal, d: ModelDraws, feats, masks, n_cat=4, opts=None, seeded_opts=None,
+                 precision=PREC_FP64) -> FitComparison:
     """Fits every held-out alignment twice on its true topology (relative
     branch lengths normalised to mean 1): from the default start and from the
     MLP's prediction.  Returns per-alignment lnL, evaluations and seconds."""
@@ -178,7 +193,7 @@ def compare_fits(prop: Proposal, d: ModelDraws, feats, masks, n_cat=4, opts=None
     pred = fit_start(prop.predict(feats))
     for k in range(n):
         rel = d.blen[k] / (d.blen[k].sum() / (len(d.blen[k]) - 1))
-        with Context() as ctx:
+        with Context(0, precision) as ctx:
             ctx.set_tips(masks[k], 4)
             ctx.set_topology(d.left[k], d.right[k], int(d.roots[k]))
             t0 = time.perf_counter()
