@@ -82,6 +82,46 @@ __device__ inline void store4(double* p, const double v[4]) {
     reinterpret_cast<double2*>(p)[1] = make_double2(v[2], v[3]);
 }
 
+// 32-bit integer hash (Wellons' "lowbias32" xorshift-multiply mix).
+__device__ inline uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// Rounding of a partial (x >= 0) to the storage type.  FP32 storage rounds
+// stochastically: 29 random bits are added below the float's last mantissa
+// bit of the FP64 value and the bits a float does not hold are cut off, so x
+// rounds up with probability equal to its distance from the float below.  A
+// subtree with few tips produces the same few values at every site, so
+// round-to-nearest would repeat the same error at thousands of sites and the
+// total lnL would drift (+4.6e-3 at 128 taxa x 100,000 sites); with
+// per-site random bits the errors average out.  The bits come from a hash of
+// (site, node, category) stepped by a Weyl sequence per state, so results
+// are reproducible and the same node rounds the same way in every program
+// that computes it.  (Zero stays zero; values below the float normal range
+// end as the nearest float of the truncated value.)
+// The dithered cut is applied to the FP64 value before the power-of-two
+// scaling (the two commute), so it overlaps the scaler exchange.
+__device__ inline double round_cut(double x, uint32_t bits, float) {
+    // 64-bit add of the 29 dither bits, written on 32-bit halves
+    const uint32_t lo = (uint32_t)__double2loint(x);
+    const uint32_t sum = lo + (bits >> 3);
+    const uint32_t hi = (uint32_t)__double2hiint(x) + (sum < lo ? 1u : 0u);
+    return __hiloint2double((int)hi, (int)(sum & 0xe0000000u));   // <= 24 significant bits
+}
+__device__ inline double round_cut(double x, uint32_t, double) { return x; }
+
+// Random bits for the partials of one (site, node, category); state i uses
+// key + i * 0x9e3779b9.  The node is identified by its two children.
+__device__ inline uint32_t round_key(int s, int l_node, int r_node, int cat) {
+    return mix32((uint32_t)s * 0x9e3779b1u ^ ((uint32_t)l_node * 0x85ebca77u + (uint32_t)r_node * 0x27d4eb2fu +
+                                              (uint32_t)cat * 0xc2b2ae3du));
+}
+
 __device__ inline double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -118,6 +158,12 @@ __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const
                                double (&xroot)[C][64]) {
     T cA[4] = {T(0), T(0), T(0), T(0)}, cB[4] = {T(0), T(0), T(0), T(0)};
     int sA = 0, sB = 0;
+    // the previous op's stored values: held in their registers until this op's
+    // exchange, so the registers a store reads are not reused while the store
+    // may still be reading them (that reuse makes the compiler wait for the
+    // store to complete before the exchange)
+    T held[4] = {T(0), T(0), T(0), T(0)};
+    int held_sc = 0;
     double lnl = 0.0;
     const T* plane = clv + ((size_t)cat * m_pad + s) * 4;          // + slot * C * m_pad * 4
     const size_t slot_stride = (size_t)C * m_pad * 4;
@@ -182,6 +228,12 @@ __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const
 #pragma unroll
             for (int i = 0; i < 4; ++i) L[i] *= R[i];
         }
+        double Lr[4];
+        {
+            const uint32_t key = round_key(s, op.l_node, op.r_node, cat);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) Lr[i] = round_cut(L[i], key + 0x9e3779b9u * i, T());
+        }
         // shared power-of-two scaler across the categories of a site
         const double mx = fmax(fmax(L[0], L[1]), fmax(L[2], L[3]));
         int e = INT_MIN;
@@ -190,6 +242,8 @@ __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const
             const int buf = k & 1;
             xexp[buf][cat][lane] = e;
             __syncthreads();
+            if (sizeof(T) == 4)
+                asm volatile("" ::"v"(held[0]), "v"(held[1]), "v"(held[2]), "v"(held[3]), "v"(held_sc));
 #pragma unroll
             for (int c = 0; c < C; ++c) e = max(e, xexp[buf][c][lane]);
         }
@@ -199,12 +253,15 @@ __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             L[i] = ldexp(L[i], -e);
-            r[i] = (T)L[i];
+            r[i] = (T)ldexp(Lr[i], -e);          // exact: Lr holds <= 24 significant bits for FP32
         }
         if (op.out_slot >= 0) {
             store4(const_cast<T*>(plane) + op.out_slot * slot_stride, r);
             if (cat == 0) scl[(size_t)op.out_slot * m_pad + s] = sc;
         }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) held[i] = r[i];
+        held_sc = sc;
         if (op.flags & OP_TO_B) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) cB[i] = r[i];
@@ -509,6 +566,13 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
 #pragma unroll
                 for (int t = 0; t < V; ++t) val[q][t] *= acc[q][t / 4][t % 4];
         }
+        double vr[G][V];
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            const uint32_t key = round_key(s[q], op.l_node, op.r_node, cat) + 0x9e3779b9u * (8u * (uint32_t)g);
+#pragma unroll
+            for (int t = 0; t < V; ++t) vr[q][t] = round_cut(val[q][t], key + 0x9e3779b9u * t, T());
+        }
         // per-site maximum over the lane groups, then over the category waves
         int e[G];
 #pragma unroll
@@ -541,7 +605,7 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
 #pragma unroll
             for (int t = 0; t < V; ++t) {
                 val[q][t] = ldexp(val[q][t], -e[q]);
-                r[t] = (T)val[q][t];
+                r[t] = (T)ldexp(vr[q][t], -e[q]);
             }
             if (op.out_slot >= 0) {
                 T* dst = clv + (size_t)op.out_slot * slot_stride + ((size_t)cat * n_sg + sg0 + q) * V * 64 + lane;

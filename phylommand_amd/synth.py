@@ -195,8 +195,7 @@ def simulate(t: Tree, Q, pi, m: int, rng: np.random.Generator, cat_rates=(1.0,))
     while i < len(order):
         v = order[i]
         i += 1
-        if t.left[v] >= 0:
-            order += [int(t.left[v]), int(t.right[v])]
+        order += [int(c) for c in (t.left[v], t.right[v]) if c >= 0]
     for v in order[1:]:
         p = t.parent[v]
         P = expm_reversible(Q, pi, t.blen[v] * cat_rates)            # [C, S, S]
