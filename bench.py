@@ -114,7 +114,8 @@ def bench_protein(args, P, synth):
     """20-state protein, 64-taxon Yule tree, 20,000 sites, 190 exchangeabilities
     ~ Gamma(1,1), frequencies ~ Dirichlet(2x20), gamma shape 1.0 (4 categories),
     3% gaps."""
-    wl = synth.gtr_gamma_workload(n_tips=64, m=20_000, S=20, C=4, alpha=1.0, gap_frac=0.03, seed=4,
+    m = args.sites or 20_000
+    wl = synth.gtr_gamma_workload(n_tips=64, m=m, S=20, C=4, alpha=1.0, gap_frac=0.03, seed=4,
                                   rate_dist="gamma", freq_conc=2.0, mean_blen=0.05)
     t = wl.tree
     ctx = P.Context(0, P.PREC_FP32)
@@ -133,7 +134,7 @@ def bench_protein(args, P, synth):
         kms.append(ctx.last_kernel_ms())
     ctx.synchronize()
     el = time.perf_counter() - t0
-    n, m, C, S = 64, 20_000, 4, 20
+    n, C, S = 64, 4, 20
     updates = (n - 1) * m * C
     ks = float(np.mean(kms)) * 1e-3
     per_site = C * S * 4 * (n - 2) + n + C * S * 4 * (n - 1) + 4 * (n - 1) + 4 * (n - 2)
@@ -143,14 +144,14 @@ def bench_protein(args, P, synth):
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic (seeded Yule tree, 20-state simulation)",
-            "config": {"workload": "20-state protein + G4, 64 taxa Yule, 20000 sites, 3% gaps", "n_taxa": n,
+            "config": {"workload": f"20-state protein + G4, 64 taxa Yule, {m} sites, 3% gaps", "n_taxa": n,
                        "n_sites": m, "n_states": S, "n_cat": C, "clv_storage": "f32", "lnL": lnl},
             # CLV x P runs on the FP64 matrix cores: 10 flop/B against a ridge of
             # 78.6 TF / 8 TB/s = 9.8 flop/B, and with 20 states padded to 32 rows
             # the MFMA roof (164 us) sits above the HBM one (101 us)
             "roofline": {"bound": "mfma", "achieved": flops * updates / ks / 1e12, "peak": FP64_MFMA_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": flops * updates / ks / 1e12 / FP64_MFMA_PEAK_TFLOPS,
-                         "traffic": profiled_traffic("prune_mfma_protein20_g4_64x20k"),
+                         "traffic": profiled_traffic("prune_mfma_protein20_g4_64x20k") if m == 20_000 else None,
                          "kernel": "prune_mfma_kernel<float,4,5>", "kernel_ms": ks * 1e3,
                          "alg_flops_per_update": flops, "alg_bytes_per_update": bpu,
                          "hbm_gbs": bpu * updates / ks / 1e9, "hbm_frac": bpu * updates / ks / 1e9 / HBM_PEAK_GBS}}
@@ -274,7 +275,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--sites", type=int, default=100_000)
+    ap.add_argument("--sites", type=int, default=None, help="sites (default 100,000 DNA, 20,000 protein)")
     ap.add_argument("--taxa", type=int, default=128)
     ap.add_argument("--fp64", action="store_true", help="store partials in FP64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -315,6 +316,7 @@ def main():
         return
 
     t0 = time.time()
+    args.sites = args.sites or 100_000
     wl = synth.gtr_gamma_workload(n_tips=args.taxa, m=args.sites, mean_blen=0.05, alpha=0.5, C=4,
                                   gap_frac=0.05, seed=2)
     gen_s = time.time() - t0

@@ -93,15 +93,23 @@ int phylo_set_model(phylo_ctx *ctx, int n_states, const double *Q, const double 
  * has no rate heterogeneity. */
 int phylo_set_categories(phylo_ctx *ctx, int n_cat, const double *rates, const double *weights);
 
-/* One full postorder evaluation; all internal-node partials are kept on the
- * device.  site_lnl (nullable): n_sites values.  total: weighted sum of the
- * site log-likelihoods (compensated FP64 sum).
+/* Whether phylo_loglik stores every internal node's partial (keep != 0) or
+ * only those its own walk reads back (keep == 0, the default: about a third
+ * of them for a Yule tree).  Results are identical either way;
+ * phylo_node_partials and phylo_nni_score run one storing evaluation first
+ * when the last one kept only part.  The reference always keeps every
+ * partial in simpleML's likelihoods map (simpleML.h:105). */
+int phylo_set_keep_partials(phylo_ctx *ctx, int keep);
+
+/* One full postorder evaluation.  site_lnl (nullable): n_sites values.
+ * total: weighted sum of the site log-likelihoods (compensated FP64 sum).
  * Replaces: simpleML::calculate_log_likelihood(sub_model&[, rate_model&])
  * (simpleML.h:58-90), calculate_likelihood (simpleML.cpp:127-184),
  * branch_likelihood (:186-197), get_log_likelihood_from_root (simpleML.h:123-141). */
 int phylo_loglik(phylo_ctx *ctx, double *site_lnl, double *total);
 
-/* Partials of one internal node from the last phylo_loglik: clv
+/* Partials of one internal node for the current inputs (those of the last
+ * phylo_loglik when nothing changed since; evaluated first if needed): clv
  * [n_sites][n_cat][n_states] (scaled), scale [n_sites] base-2 exponents so
  * that the true partial is clv * 2^scale.  Replaces: reading simpleML's
  * likelihoods map in draw_normalized_likelihood_on_nodes (simpleML.cpp:199-231). */

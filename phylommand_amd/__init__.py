@@ -34,6 +34,7 @@ SIGNATURES = [
     ("phylo_destroy", None, [_c_void_p]),
     ("phylo_last_error", ctypes.c_char_p, [_c_void_p]),
     ("phylo_set_precision", _c_int, [_c_void_p, _c_int]),
+    ("phylo_set_keep_partials", _c_int, [_c_void_p, _c_int]),
     ("phylo_set_tips", _c_int, [_c_void_p, _c_int, _c_int, _c_int, ctypes.POINTER(ctypes.c_uint32)]),
     ("phylo_set_site_weights", _c_int, [_c_void_p, _c_dbl_p]),
     ("phylo_set_topology", _c_int, [_c_void_p, _c_int, _c_int, _c_int_p, _c_int_p]),
@@ -214,6 +215,12 @@ class Context:
 
     def set_precision(self, precision):
         self._check(self.lib.phylo_set_precision(self.h, precision))
+
+    def set_keep_partials(self, keep: bool):
+        """Store every internal node's partial in loglik (default: only those
+        the walk reads back; node_partials / nni_score evaluate again when
+        they need the rest)."""
+        self._check(self.lib.phylo_set_keep_partials(self.h, int(bool(keep))))
 
     def set_tips(self, masks: np.ndarray, n_states: int):
         masks = np.ascontiguousarray(masks, dtype=np.uint32)

@@ -819,7 +819,8 @@ struct phylo_ctx {
     int* d_nni_prog = nullptr; size_t nni_prog_bytes = 0;
     const Op* run_ops = nullptr;                          // program set of the launch in flight
     const int* run_prog = nullptr;
-    bool clv_valid = false;                               // stored partials match the current inputs
+    bool clv_valid = false;                               // every stored partial matches the current inputs
+    bool keep_partials = false;                           // loglik stores every node's partial
     double* d_roots = nullptr;
     double* d_sitew = nullptr; size_t sitew_bytes = 0;
     double* d_sitelnl = nullptr; size_t sitelnl_bytes = 0;
@@ -955,10 +956,38 @@ bool build_full_ops(const TV* c, std::vector<Op>& ops) {
     return true;
 }
 
+// Clears the stores of ops[begin, end) whose result no op of that range reads
+// back from memory: results consumed only through the register carry.  In a
+// postorder walk with one carry only nodes with two internal children keep a
+// store (40 of the 127 internal nodes of a 128-taxon Yule tree), so the
+// partials written per evaluation drop by about two thirds.
+void drop_unread_stores(std::vector<Op>& ops, size_t begin, size_t end) {
+    std::vector<char> read;
+    for (size_t k = begin; k < end; ++k) {
+        const Op& o = ops[k];
+        for (int side = 0; side < 2; ++side) {
+            const int kind = side ? o.r_kind : o.l_kind, idx = side ? o.r_idx : o.l_idx;
+            if (kind != SRC_CLV) continue;
+            if ((size_t)idx >= read.size()) read.resize(idx + 1, 0);
+            read[idx] = 1;
+        }
+    }
+    for (size_t k = begin; k < end; ++k) {
+        Op& o = ops[k];
+        if (o.out_slot >= 0 && ((size_t)o.out_slot >= read.size() || !read[o.out_slot])) o.out_slot = -1;
+    }
+}
+
+// Two programs over the same postorder: 0 stores every internal node's
+// partial (needed by node_partials and the NNI partial recompute), 1 stores
+// only what the walk itself reads back.  Both compute identical values.
 int compile_full_program(phylo_ctx* c) {
     c->ops.clear();
     if (!build_full_ops(c, c->ops)) return fail(c, PHYLO_ERR_ARG, "topology: internal nodes unreachable from the root");
-    c->prog_begin = {0, (int)c->ops.size()};
+    const size_t n = c->ops.size();
+    c->ops.insert(c->ops.end(), c->ops.begin(), c->ops.end());
+    drop_unread_stores(c->ops, n, 2 * n);
+    c->prog_begin = {0, (int)n, (int)(2 * n)};
     c->n_slots = c->n_nodes - c->n_taxa;
     return PHYLO_OK;
 }
@@ -1516,23 +1545,44 @@ int phylo_set_categories(phylo_ctx* c, int n_cat, const double* rates, const dou
     return PHYLO_OK;
 }
 
-int phylo_loglik(phylo_ctx* c, double* site_lnl, double* total) {
-    if (!c) return PHYLO_ERR_ARG;
-    if (!total) return fail(c, PHYLO_ERR_ARG, "loglik: total is required");
+namespace {
+// One evaluation of the current inputs; `full` stores every internal node's
+// partial, otherwise only those the walk reads back.
+int evaluate(phylo_ctx* c, bool full, double* site_lnl, double* total) {
     int rc = prepare(c);
     if (rc) return rc;
     c->clv_valid = false;
-    rc = run_programs(c, c->d_ops, c->d_prog, 1, total, site_lnl);
+    rc = run_programs(c, c->d_ops, c->d_prog + (full ? 0 : 1), 1, total, site_lnl);
     if (rc) return rc;
-    c->clv_valid = true;
+    c->clv_valid = full;
     c->base_total = *total;
     return PHYLO_OK;
+}
+}  // namespace
+
+int phylo_set_keep_partials(phylo_ctx* c, int keep) {
+    if (!c) return PHYLO_ERR_ARG;
+    c->keep_partials = keep != 0;
+    return PHYLO_OK;
+}
+
+int phylo_loglik(phylo_ctx* c, double* site_lnl, double* total) {
+    if (!c) return PHYLO_ERR_ARG;
+    if (!total) return fail(c, PHYLO_ERR_ARG, "loglik: total is required");
+    return evaluate(c, c->keep_partials, site_lnl, total);
 }
 
 int phylo_node_partials(phylo_ctx* c, int node, double* clv, int32_t* scale) {
     if (!c) return PHYLO_ERR_ARG;
     if (!c->d_clv || c->ops.empty()) return fail(c, PHYLO_ERR_STATE, "node_partials before loglik");
     if (node < c->n_taxa || node >= c->n_nodes) return fail(c, PHYLO_ERR_ARG, "node_partials: not an internal node");
+    if (!c->clv_valid) {
+        // the last evaluation stored only what its walk read back (or the
+        // inputs changed since): one evaluation that stores every partial
+        double t = 0.0;
+        int rc = evaluate(c, true, nullptr, &t);
+        if (rc) return rc;
+    }
     const int slot = node - c->n_taxa;
     const int Sp = clv_states(c);
     const size_t n = (size_t)c->C * c->m_pad * Sp;
@@ -1605,7 +1655,7 @@ int phylo_nni_score(phylo_ctx* c, int mode, int max_variants, double* lnl, int32
     if (rc) return rc;
     double base = 0.0;
     if (!c->clv_valid) {
-        rc = phylo_loglik(c, nullptr, &base);
+        rc = evaluate(c, true, nullptr, &base);   // the variants read every stored partial
         if (rc) return rc;
     }
     base = c->base_total;
@@ -2000,6 +2050,7 @@ int phylo_batch_finalize(phylo_batch* b, int autotune) {
         roots_elems += kMaxStates + kMaxCat;
         const size_t before = ops.size();
         if (!build_full_ops(&I, ops)) return fail(b, PHYLO_ERR_ARG, "batch: internal nodes unreachable from the root");
+        if (I.nni_mode < 0) drop_unread_stores(ops, before, ops.size());   // no variant reads the partials
         I.full_prog = (int)prog_begin.size() - 1;
         prog_begin.push_back((int)ops.size());
         prog_len.push_back((int)(ops.size() - before));

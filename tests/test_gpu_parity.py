@@ -87,3 +87,49 @@ def test_node_partials_match_oracle(oracle, n, m, S, C, prec):
             scale = np.abs(want).max(axis=(1, 2), keepdims=True)
             tol = 1e-10 if prec == 1 else 2e-5
             np.testing.assert_allclose(got / scale, want / scale, atol=tol, err_msg=f"node {v}")
+
+
+@pytest.mark.parametrize("S,prec", [(4, 0), (4, 1), (20, 0)])
+def test_partial_storing_does_not_change_results(S, prec):
+    """loglik stores only the partials its walk reads back unless asked to
+    keep them all: totals, per-site values, node partials (completed by one
+    storing evaluation) and NNI scores agree bit for bit with the
+    keep-everything evaluation."""
+    print(f"keep partials S={S} prec={prec}", flush=True)
+    kw = dict(rate_dist="gamma", freq_conc=2.0) if S > 4 else {}
+    wl = synth.gtr_gamma_workload(n_tips=24, m=517, S=S, C=4, alpha=0.7, gap_frac=0.05, seed=40 + S, **kw)
+    t = wl.tree
+    out = []
+    for keep in (False, True):
+        with gpu_context(wl, precision=prec) as ctx:
+            ctx.set_keep_partials(keep)
+            tot, site = ctx.loglik(per_site=True)
+            parts = [ctx.node_partials(v, 4, S) for v in range(t.n_tips, t.n_nodes)]
+            base, var = ctx.nni_score(1)
+            tot2 = ctx.loglik()
+            out.append((tot, site, parts, base, var, tot2))
+    (a, b) = out
+    assert a[0] == b[0] and a[5] == b[5] == a[0]
+    np.testing.assert_array_equal(a[1], b[1])
+    for (ca, sa), (cb, sb) in zip(a[2], b[2]):
+        np.testing.assert_array_equal(ca, cb)
+        np.testing.assert_array_equal(sa, sb)
+    assert a[3] == b[3] == a[0]
+    np.testing.assert_array_equal(a[4], b[4])
+
+
+def test_node_partials_follow_changed_inputs(oracle):
+    """node_partials after the edge lengths change evaluates the new inputs."""
+    wl = synth.gtr_gamma_workload(n_tips=10, m=200, C=4, alpha=0.8, gap_frac=0.05, seed=77)
+    t = wl.tree
+    eff2 = t.blen * 1.7
+    clv_o, sc_o, _, _ = oracle_problem(oracle, wl, eff=eff2).node_clvs()
+    with gpu_context(wl, precision=1) as ctx:
+        ctx.loglik()
+        ctx.set_edge_lengths(eff2)
+        for v in range(t.n_tips, t.n_nodes):
+            clv, sc = ctx.node_partials(v, 4, 4)
+            got = clv * np.ldexp(1.0, sc)[:, None, None]
+            want = clv_o[v] * np.ldexp(1.0, sc_o[v])[:, None, None]
+            scale = np.abs(want).max(axis=(1, 2), keepdims=True)
+            np.testing.assert_allclose(got / scale, want / scale, atol=1e-10, err_msg=f"node {v}")
