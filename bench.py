@@ -99,6 +99,76 @@ def cpu_baseline(wl, budget_s=20.0):
             "sample": f"oracle restatement of simpleML's cost model, {n} taxa x {m_sample} sites, 1 category"}
 
 
+_FAIR_CHILD = r"""
+import sys, time
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from oracle import oracle as O
+d = np.load(sys.argv[2])
+k, ms = int(sys.argv[3]), int(sys.argv[4])
+mb = np.ascontiguousarray(d["masks_by_node"][:, k * ms:(k + 1) * ms])
+p = O.Problem(d["left"], d["right"], int(d["root"]), d["blen"], mb, 4, d["Q"], d["pi"], pi=d["pi"],
+              cat_rates=d["cat_rates"], cat_w=d["cat_w"])
+best = 1e30
+for _ in range(3):
+    t0 = time.perf_counter()
+    p.loglik()
+    best = min(best, time.perf_counter() - t0)
+print(best)
+"""
+
+
+def fair_cpu(wl, cores, m_shard=20_000):
+    """A CPU path without the reference's per-site P recomputation: the
+    oracle's FP64 pruning (P built once per edge and category, -O2 C) with
+    gamma categories, one process per core on independent site shards."""
+    t = wl.tree
+    n = t.n_tips
+    m_shard = min(m_shard, wl.masks.shape[1] // cores)
+    mb = np.zeros((t.n_nodes, m_shard * cores), np.uint32)
+    mb[:n] = wl.masks[:, :m_shard * cores]
+    with tempfile.TemporaryDirectory() as d:
+        f = os.path.join(d, "wl.npz")
+        np.savez(f, masks_by_node=mb, left=t.left, right=t.right, root=t.root, blen=t.blen, Q=wl.Q, pi=wl.pi,
+                 cat_rates=wl.cat_rates, cat_w=wl.cat_weights)
+        t0 = time.time()
+        procs = [subprocess.Popen([sys.executable, "-c", _FAIR_CHILD, ROOT, f, str(k), str(m_shard)],
+                                  stdout=subprocess.PIPE, text=True) for k in range(cores)]
+        secs = [float(p.communicate(timeout=600)[0].strip().split()[-1]) for p in procs]
+        wall = time.time() - t0
+    C = len(wl.cat_rates)
+    per = (n - 1) * m_shard * C
+    return {"value": sum(per / s for s in secs), "unit": "updates/s", "cores": cores,
+            "single_process": per / max(secs),
+            "sample": f"oracle FP64 pruning (P once per edge and category, gcc -O2), {n} taxa x {m_shard} sites "
+                      f"x {C} categories per process, {cores} concurrent site shards, best of 3; {wall:.1f}s wall"}
+
+
+def copy_bandwidth(mib=1024, reps=20):
+    """Device-to-device copy rate (bytes read + written per second) of a
+    plain 1 GiB copy on this GPU, timed with events on torch's stream."""
+    import torch
+    a = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    sec = e0.elapsed_time(e1) * 1e-3 / reps
+    return 2 * a.numel() * 4 / sec / 1e9
+
+
+def copy_bandwidth_subprocess():
+    """copy_bandwidth in a child process that loads only torch's HIP runtime."""
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-probe", "1024", "--copy-time"],
+                       capture_output=True, text=True, timeout=300)
+    return float(json.loads(r.stdout.strip().splitlines()[-1])["copy_gbs"])
+
+
 def profiled_traffic(name):
     """HBM bytes per launch of a kernel on its bench workload, from the
     FETCH_SIZE / WRITE_SIZE counter passes summarised in profiles/<name>.md
@@ -286,7 +356,11 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run only a plain device-to-device copy of MIB MiB (torch), the calibration kernel "
                          "for the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--copy-time", action="store_true", help="with --copy-probe: time the copy, print GB/s")
     args = ap.parse_args()
+    if args.copy_probe and args.copy_time:
+        print(json.dumps({"copy_gbs": copy_bandwidth(args.copy_probe)}))
+        return
     if args.copy_probe:
         import torch
         a = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda")
@@ -369,11 +443,22 @@ def main():
                      "kernel": "prune4_kernel<float,4>", "kernel_ms": kernel_s * 1e3,
                      "alg_bytes_per_update": bpu},
     }
+    try:
+        cgbs = copy_bandwidth_subprocess()
+        out["roofline"]["copy_gbs"] = cgbs
+        out["roofline"]["frac_of_copy"] = achieved / cgbs
+    except Exception as e:  # optional calibration
+        out["roofline"]["copy_gbs"] = None
+        print(f"[bench] copy bandwidth not measured: {e}", file=sys.stderr)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(wl)
         except Exception as e:  # the GPU figure stands on its own
             out["cpu_baseline"] = {"value": None, "error": str(e)}
+        try:
+            out["cpu_baseline"]["fair_cpu"] = fair_cpu(wl, out["cpu_baseline"].get("cores") or 16)
+        except Exception as e:
+            out["cpu_baseline"]["fair_cpu"] = {"value": None, "error": str(e)}
     print(json.dumps(out))
 
 

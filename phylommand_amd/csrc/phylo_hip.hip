@@ -104,8 +104,9 @@ __device__ inline uint32_t mix32(uint32_t x) {
 // are reproducible and the same node rounds the same way in every program
 // that computes it.  (Zero stays zero; values below the float normal range
 // end as the nearest float of the truncated value.)
-// The dithered cut is applied to the FP64 value before the power-of-two
-// scaling (the two commute), so it overlaps the scaler exchange.
+// The dithered cut and the power-of-two scaling commute (the cut reads only
+// mantissa bits), so a walker may cut before the scaler exchange, to overlap
+// it, or after it, on the few results it stores.
 __device__ inline double round_cut(double x, uint32_t bits, float) {
     // 64-bit add of the 29 dither bits, written on 32-bit halves
     const uint32_t lo = (uint32_t)__double2loint(x);
@@ -155,8 +156,11 @@ template <typename T, int C>
 __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const uint8_t* __restrict__ tips,
                                T* clv, int* scl, const double* __restrict__ P, const double* __restrict__ roots,
                                int m_pad, int s, int lane, int cat, int (&xexp)[2][C][64],
-                               double (&xroot)[C][64]) {
-    T cA[4] = {T(0), T(0), T(0), T(0)}, cB[4] = {T(0), T(0), T(0), T(0)};
+                               double (&xroot)[C][64], double2 (&xb)[C][2][64]) {
+    // carried results stay in FP64: only stored partials are rounded to T
+    // (carry B, the one result waiting while a sibling subtree is walked,
+    // sits in LDS: xb[cat][][lane], written and read by the same lane)
+    double cA[4] = {0.0, 0.0, 0.0, 0.0};
     int sA = 0, sB = 0;
     // the previous op's stored values: held in their registers until this op's
     // exchange, so the registers a store reads are not reused while the store
@@ -185,11 +189,13 @@ __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const
                 sc += scl[(size_t)op.l_idx * m_pad + s];
             } else if (op.l_kind == SRC_CARRY_A) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) x[j] = (double)cA[j];
+                for (int j = 0; j < 4; ++j) x[j] = cA[j];
                 sc += sA;
             } else {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) x[j] = (double)cB[j];
+                {
+                    const double2 a = xb[cat][0][lane], b = xb[cat][1][lane];
+                    x[0] = a.x; x[1] = a.y; x[2] = b.x; x[3] = b.y;
+                }
                 sc += sB;
             }
 #pragma unroll
@@ -211,11 +217,13 @@ __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const
                 sc += scl[(size_t)op.r_idx * m_pad + s];
             } else if (op.r_kind == SRC_CARRY_A) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) x[j] = (double)cA[j];
+                for (int j = 0; j < 4; ++j) x[j] = cA[j];
                 sc += sA;
             } else {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) x[j] = (double)cB[j];
+                {
+                    const double2 a = xb[cat][0][lane], b = xb[cat][1][lane];
+                    x[0] = a.x; x[1] = a.y; x[2] = b.x; x[3] = b.y;
+                }
                 sc += sB;
             }
 #pragma unroll
@@ -227,12 +235,6 @@ __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i) L[i] *= R[i];
-        }
-        double Lr[4];
-        {
-            const uint32_t key = round_key(s, op.l_node, op.r_node, cat);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) Lr[i] = round_cut(L[i], key + 0x9e3779b9u * i, T());
         }
         // shared power-of-two scaler across the categories of a site
         const double mx = fmax(fmax(L[0], L[1]), fmax(L[2], L[3]));
@@ -249,26 +251,28 @@ __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const
         }
         if (e == INT_MIN) e = 0;
         sc += e;
-        T r[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            L[i] = ldexp(L[i], -e);
-            r[i] = (T)ldexp(Lr[i], -e);          // exact: Lr holds <= 24 significant bits for FP32
-        }
+        for (int i = 0; i < 4; ++i) L[i] = ldexp(L[i], -e);
         if (op.out_slot >= 0) {
+            // most results are only carried (the program stores what it reads
+            // back), so the rounding sits here, on the few stored ones
+            const uint32_t key = round_key(s, op.l_node, op.r_node, cat);
+            T r[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) r[i] = (T)round_cut(L[i], key + 0x9e3779b9u * i, T());
             store4(const_cast<T*>(plane) + op.out_slot * slot_stride, r);
             if (cat == 0) scl[(size_t)op.out_slot * m_pad + s] = sc;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) held[i] = r[i];
+            held_sc = sc;
         }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) held[i] = r[i];
-        held_sc = sc;
         if (op.flags & OP_TO_B) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) cB[i] = r[i];
+            xb[cat][0][lane] = make_double2(L[0], L[1]);
+            xb[cat][1][lane] = make_double2(L[2], L[3]);
             sB = sc;
         } else {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) cA[i] = r[i];
+            for (int i = 0; i < 4; ++i) cA[i] = L[i];
             sA = sc;
         }
         if (op.flags & OP_ROOT) {
@@ -299,12 +303,13 @@ __global__ __launch_bounds__(64 * C) void prune4_kernel(
     int m, int m_pad) {
     __shared__ int xexp[2][C][64];
     __shared__ double xroot[C][64];
+    __shared__ double2 xb[C][2][64];
     const int lane = threadIdx.x & 63;
     const int cat = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int s = blockIdx.x * 64 + lane;
     const int prog = blockIdx.y;
     const double lnl = walk4<T, C>(ops, prog_begin[prog], prog_begin[prog + 1], tips, clv, scl, P, roots, m_pad, s,
-                                   lane, cat, xexp, xroot);
+                                   lane, cat, xexp, xroot, xb);
     const bool live = s < m && cat == 0;
     if (live && site_lnl) site_lnl[(size_t)prog * m + s] = lnl;
     const double w = live ? (site_w ? site_w[s] : 1.0) : 0.0;
@@ -342,6 +347,7 @@ __global__ __launch_bounds__(64 * C) void batch4_kernel(const WorkItem* __restri
                                                         double* __restrict__ partial, int blocks_per_item) {
     __shared__ int xexp[2][C][64];
     __shared__ double xroot[C][64];
+    __shared__ double2 xb[C][2][64];
     const int lane = threadIdx.x & 63;
     const int cat = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
@@ -353,7 +359,8 @@ __global__ __launch_bounds__(64 * C) void batch4_kernel(const WorkItem* __restri
             const int s = (w.block + q) * 64 + lane;
             if ((w.block + q) * 64 >= I.m_pad) break;
             const double lnl = walk4<T, C>(ops, k0, k1, tips + I.tips_off, clv + I.clv_off, scl + I.scl_off,
-                                           P + I.P_off, roots + I.roots_off, I.m_pad, s, lane, cat, xexp, xroot);
+                                           P + I.P_off, roots + I.roots_off, I.m_pad, s, lane, cat, xexp, xroot,
+                                           xb);
             if (s < I.m && cat == 0) acc += lnl;
         }
         block_sum_store(acc, partial + w.out);
@@ -533,6 +540,8 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
     const int prog = blockIdx.y;
     const int k0 = prog_begin[prog], k1 = prog_begin[prog + 1];
     const size_t slot_stride = (size_t)C * m_pad * 4 * V;
+    // carried results are rounded to T like stored ones (registers are the
+    // limit of this walker's occupancy)
     T cA[G][V], cB[G][V];
     int sA[G], sB[G];
 #pragma unroll
@@ -913,34 +922,66 @@ int padded_states(int S) {
 int clv_states(const phylo_ctx* c) { return c->S > 4 ? 4 * mfma_slots(c->S) : 4; }
 bool uses_mfma(const phylo_ctx* c) { return c->S > 4; }
 
-// Appends the full-evaluation program of a tree: internal nodes in the
-// reference's postorder (left subtree, right subtree, node), carry A holding
-// the previous op's result.  Returns false if internal nodes are unreachable.
+// Postorder of the internal nodes below `root`, visiting first[v] before
+// second[v] (either may be -1).  Returns the nodes visited.
 template <class TV>
-bool build_full_ops(const TV* c, std::vector<Op>& ops) {
-    const int n = c->n_nodes;
+std::vector<int> internal_postorder(const TV* c, const std::vector<int>& first, const std::vector<int>& second) {
     std::vector<int> order;
-    order.reserve(n);
+    order.reserve(c->n_nodes);
     std::vector<int> stack{c->root};
-    std::vector<char> seen(n, 0);
+    std::vector<char> seen(c->n_nodes, 0);
     while (!stack.empty()) {
-        int v = stack.back();
+        const int v = stack.back();
         if (!seen[v]) {
             seen[v] = 1;
-            if (c->right[v] >= 0) stack.push_back(c->right[v]);
-            if (c->left[v] >= 0) stack.push_back(c->left[v]);
+            if (second[v] >= 0) stack.push_back(second[v]);
+            if (first[v] >= 0) stack.push_back(first[v]);
         } else {
             stack.pop_back();
             if (v >= c->n_taxa) order.push_back(v);
         }
     }
-    if ((int)order.size() != n - c->n_taxa) return false;
-    int prev = -1;
+    return order;
+}
+
+// Appends the full-evaluation program of a tree: every internal node once,
+// children before parents, each op's result kept in a register for the next
+// op (carry A).  A node with two internal children needs one child's result
+// while the other child's subtree is walked; that result waits in a second
+// register (carry B) when the other subtree has no such node itself, and in
+// memory otherwise.  The subtree without branching goes second where one
+// exists, so that as many waits as possible use the register: with two
+// registers a 128-taxon Yule tree keeps 11 of 127 partials in memory instead
+// of 40.  The order of the walk does not change any value (each op computes
+// (P_l L_l) x (P_r L_r) of the same children); the reference's recursion is
+// left subtree, right subtree, node (simpleML.cpp:127-184).
+// Returns false if internal nodes are unreachable.
+template <class TV>
+bool build_full_ops(const TV* c, std::vector<Op>& ops) {
+    const int n = c->n_nodes, nt = c->n_taxa;
+    std::vector<int> first(c->left.begin(), c->left.begin() + n), second(c->right.begin(), c->right.begin() + n);
+    const std::vector<int> post = internal_postorder(c, first, second);
+    if ((int)post.size() != n - nt) return false;
+    const auto internal = [&](int x) { return x >= nt; };
+    // branchy[v]: v's subtree holds a node with two internal children
+    std::vector<char> branchy(n, 0), to_b(n, 0);
+    for (int v : post) {
+        const int l = c->left[v], r = c->right[v];
+        const bool two = internal(l) && r >= 0 && internal(r);
+        branchy[v] = two || (l >= 0 && branchy[l]) || (r >= 0 && branchy[r]);
+        if (two) {
+            if (branchy[r] && !branchy[l]) std::swap(first[v], second[v]);
+            to_b[first[v]] = !branchy[second[v]];
+        }
+    }
+    const std::vector<int> order = internal_postorder(c, first, second);
+    int prev = -1, in_b = -1;     // node whose result is in carry A / carry B
     auto src = [&](int child, int& kind, int& idx) {
         if (child < 0) { kind = SRC_NONE; idx = 0; }
-        else if (child < c->n_taxa) { kind = SRC_TIP; idx = child; }
+        else if (child < nt) { kind = SRC_TIP; idx = child; }
         else if (child == prev) { kind = SRC_CARRY_A; idx = 0; }
-        else { kind = SRC_CLV; idx = child - c->n_taxa; }
+        else if (child == in_b) { kind = SRC_CARRY_B; idx = 0; }
+        else { kind = SRC_CLV; idx = child - nt; }
     };
     for (int v : order) {
         Op op{};
@@ -948,10 +989,17 @@ bool build_full_ops(const TV* c, std::vector<Op>& ops) {
         op.r_node = c->right[v] < 0 ? 0 : c->right[v];
         src(c->left[v], op.l_kind, op.l_idx);
         src(c->right[v], op.r_kind, op.r_idx);
-        op.out_slot = v - c->n_taxa;
+        if (op.l_kind == SRC_CARRY_B || op.r_kind == SRC_CARRY_B) in_b = -1;
+        op.out_slot = v - nt;
         op.flags = (v == c->root) ? OP_ROOT : 0;
+        if (to_b[v] && v != c->root) {
+            op.flags |= OP_TO_B;
+            in_b = v;
+            prev = -1;
+        } else {
+            prev = v;
+        }
         ops.push_back(op);
-        prev = v;
     }
     return true;
 }
