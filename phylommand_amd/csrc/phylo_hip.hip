@@ -152,11 +152,13 @@ __device__ inline void block_sum_store(double v, double* dst) {
 // its per-site maximum, the proposals meet in LDS (double-buffered, one
 // barrier per op) and every wave applies the largest.
 // walk4 runs ops[k0, k1) for site s and returns the site's lnL (root op).
-template <typename T, int C>
+// With TL the tip codes of the workgroup's 64 sites come from LDS (tl: taxon
+// t's sites 2b and 2b+1 in the low and high nibble of tl[32 t + b]).
+template <typename T, int C, bool TL>
 __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const uint8_t* __restrict__ tips,
-                               T* clv, int* scl, const double* __restrict__ P, const double* __restrict__ roots,
-                               int m_pad, int s, int lane, int cat, int (&xexp)[2][C][64],
-                               double (&xroot)[C][64], double2 (&xb)[C][2][64]) {
+                               const uint8_t* tl, T* clv, int* scl, const double* __restrict__ P,
+                               const double* __restrict__ roots, int m_pad, int s, int lane, int cat,
+                               int (&xexp)[2][C][64], double (&xroot)[C][64], double2 (&xb)[C][2][64]) {
     // carried results stay in FP64: only stored partials are rounded to T
     // (carry B, the one result waiting while a sibling subtree is walked,
     // sits in LDS: xb[cat][][lane], written and read by the same lane)
@@ -181,7 +183,8 @@ __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const
         {
             double x[4];
             if (op.l_kind == SRC_TIP) {
-                const unsigned code = tips[(size_t)op.l_idx * m_pad + s];
+                const unsigned code = TL ? (tl[op.l_idx * 32 + (lane >> 1)] >> ((lane & 1) * 4)) & 15u
+                                         : tips[(size_t)op.l_idx * m_pad + s];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) x[j] = (code >> j) & 1u ? 1.0 : 0.0;
             } else if (op.l_kind == SRC_CLV) {
@@ -209,7 +212,8 @@ __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const
         if (op.r_kind != SRC_NONE) {
             double x[4];
             if (op.r_kind == SRC_TIP) {
-                const unsigned code = tips[(size_t)op.r_idx * m_pad + s];
+                const unsigned code = TL ? (tl[op.r_idx * 32 + (lane >> 1)] >> ((lane & 1) * 4)) & 15u
+                                         : tips[(size_t)op.r_idx * m_pad + s];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) x[j] = (code >> j) & 1u ? 1.0 : 0.0;
             } else if (op.r_kind == SRC_CLV) {
@@ -294,22 +298,36 @@ __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const
     return lnl;
 }
 
+// Taxa whose tip codes a DNA workgroup stages in LDS (4 bits per site).
+constexpr int kTipLdsTaxa = 256;
+
 // grid = (m_pad / 64, n_programs); program y runs ops[prog_begin[y] .. prog_begin[y+1]).
-template <typename T, int C>
+// TL (n_taxa <= kTipLdsTaxa): the workgroup first copies its 64 sites of
+// every tip into LDS, so no op of the walk waits on a global tip load.
+template <typename T, int C, bool TL>
 __global__ __launch_bounds__(64 * C) void prune4_kernel(
     const Op* __restrict__ ops, const int* __restrict__ prog_begin, const uint8_t* __restrict__ tips,
     T* clv, int* scl, const double* __restrict__ P, const double* __restrict__ roots,
     const double* __restrict__ site_w, double* __restrict__ site_lnl, double* __restrict__ partial,
-    int m, int m_pad) {
+    int m, int m_pad, int n_taxa) {
     __shared__ int xexp[2][C][64];
     __shared__ double xroot[C][64];
     __shared__ double2 xb[C][2][64];
+    __shared__ uint8_t tl[TL ? kTipLdsTaxa * 32 : 1];
     const int lane = threadIdx.x & 63;
     const int cat = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int s = blockIdx.x * 64 + lane;
     const int prog = blockIdx.y;
-    const double lnl = walk4<T, C>(ops, prog_begin[prog], prog_begin[prog + 1], tips, clv, scl, P, roots, m_pad, s,
-                                   lane, cat, xexp, xroot, xb);
+    if (TL) {
+        const uint8_t* blk = tips + (size_t)blockIdx.x * 64;
+        for (int i = threadIdx.x; i < n_taxa * 32; i += 64 * C) {
+            const uint32_t pr = *reinterpret_cast<const uint16_t*>(blk + (size_t)(i >> 5) * m_pad + 2 * (i & 31));
+            tl[i] = (uint8_t)((pr & 15u) | ((pr >> 4) & 0xF0u));
+        }
+        __syncthreads();
+    }
+    const double lnl = walk4<T, C, TL>(ops, prog_begin[prog], prog_begin[prog + 1], tips, tl, clv, scl, P, roots,
+                                       m_pad, s, lane, cat, xexp, xroot, xb);
     const bool live = s < m && cat == 0;
     if (live && site_lnl) site_lnl[(size_t)prog * m + s] = lnl;
     const double w = live ? (site_w ? site_w[s] : 1.0) : 0.0;
@@ -358,7 +376,8 @@ __global__ __launch_bounds__(64 * C) void batch4_kernel(const WorkItem* __restri
         for (int q = 0; q < blocks_per_item; ++q) {
             const int s = (w.block + q) * 64 + lane;
             if ((w.block + q) * 64 >= I.m_pad) break;
-            const double lnl = walk4<T, C>(ops, k0, k1, tips + I.tips_off, clv + I.clv_off, scl + I.scl_off,
+            const double lnl = walk4<T, C, false>(ops, k0, k1, tips + I.tips_off, nullptr, clv + I.clv_off,
+                                                  scl + I.scl_off,
                                            P + I.P_off, roots + I.roots_off, I.m_pad, s, lane, cat, xexp, xroot,
                                            xb);
             if (s < I.m && cat == 0) acc += lnl;
@@ -828,6 +847,7 @@ struct phylo_ctx {
     int* d_nni_prog = nullptr; size_t nni_prog_bytes = 0;
     const Op* run_ops = nullptr;                          // program set of the launch in flight
     const int* run_prog = nullptr;
+    bool run_whole_tree = false;                          // the launch in flight walks every internal node
     bool clv_valid = false;                               // every stored partial matches the current inputs
     bool keep_partials = false;                           // loglik stores every node's partial
     double* d_roots = nullptr;
@@ -1181,10 +1201,18 @@ int prepare(phylo_ctx* c) {
 
 template <typename T, int C>
 void launch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
-    hipLaunchKernelGGL((prune4_kernel<T, C>), grid, dim3(64 * C), 0, c->stream, c->run_ops, c->run_prog,
-                       reinterpret_cast<const uint8_t*>(c->d_tips),
-                       reinterpret_cast<T*>(c->d_clv), c->d_scl, c->d_P, c->d_roots,
-                       c->have_site_w ? c->d_sitew : nullptr, site_lnl, c->d_partial, c->m, c->m_pad);
+    const auto* tips = reinterpret_cast<const uint8_t*>(c->d_tips);
+    T* clv = reinterpret_cast<T*>(c->d_clv);
+    const double* sw = c->have_site_w ? c->d_sitew : nullptr;
+    // staging every tip pays off for whole-tree walks, not for the short
+    // path-to-root programs of an NNI neighbourhood
+    if (c->run_whole_tree && c->n_taxa <= kTipLdsTaxa)
+        hipLaunchKernelGGL((prune4_kernel<T, C, true>), grid, dim3(64 * C), 0, c->stream, c->run_ops, c->run_prog, tips,
+                           clv, c->d_scl, c->d_P, c->d_roots, sw, site_lnl, c->d_partial, c->m, c->m_pad, c->n_taxa);
+    else
+        hipLaunchKernelGGL((prune4_kernel<T, C, false>), grid, dim3(64 * C), 0, c->stream, c->run_ops, c->run_prog,
+                           tips, clv, c->d_scl, c->d_P, c->d_roots, sw, site_lnl, c->d_partial, c->m, c->m_pad,
+                           c->n_taxa);
 }
 
 template <typename T>
@@ -1600,7 +1628,9 @@ int evaluate(phylo_ctx* c, bool full, double* site_lnl, double* total) {
     int rc = prepare(c);
     if (rc) return rc;
     c->clv_valid = false;
+    c->run_whole_tree = true;
     rc = run_programs(c, c->d_ops, c->d_prog + (full ? 0 : 1), 1, total, site_lnl);
+    c->run_whole_tree = false;
     if (rc) return rc;
     c->clv_valid = full;
     c->base_total = *total;
