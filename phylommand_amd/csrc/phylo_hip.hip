@@ -468,10 +468,11 @@ __device__ inline double to_d(double x) { return x; }
 // One child for the G site groups of a wave: acc[q][T] = P . x for tile T,
 // x in operand order.  The P operand of each K-step is loaded once and used by
 // every group.
-template <typename T, int C, int V, int G>
+template <typename T, int C, int V, int G, bool TL>
 __device__ inline void mfma_child(int kind, int idx, int node, int cat, const int (&s)[G], int sg0, int n_sg,
                                   int lane, int g, int m_pad, int S, uint32_t full,
-                                  const uint32_t* __restrict__ tips, const T* clv, size_t slot_stride,
+                                  const uint32_t* __restrict__ tips, const uint32_t* tl, const T* clv,
+                                  size_t slot_stride,
                                   const int* scl, const double* __restrict__ Pm, const double* __restrict__ PTm,
                                   const T (&cA)[G][V], const int (&sA)[G], const T (&cB)[G][V], const int (&sB)[G],
                                   mfma_d4 (&acc)[G][2], int (&sc)[G]) {
@@ -482,7 +483,7 @@ __device__ inline void mfma_child(int kind, int idx, int node, int cat, const in
         bool simple = true;
 #pragma unroll
         for (int q = 0; q < G; ++q) {
-            mk[q] = tips[(size_t)idx * m_pad + s[q]];
+            mk[q] = TL ? tl[idx * 16 * G + 16 * q + (lane & 15)] : tips[(size_t)idx * m_pad + s[q]];
             simple = simple && (mk[q] == full || __popc(mk[q]) == 1);
         }
         // a wave whose sites are all single states or gaps selects columns of P
@@ -541,14 +542,26 @@ __device__ inline void mfma_child(int kind, int idx, int node, int cat, const in
 
 // grid = (m_pad / (16 G), n_programs), block = 64 * C: wave c computes
 // category c of the G 16-site groups starting at group G * blockIdx.x.
-template <typename T, int C, int V, int G>
+// Taxa whose tip masks a wide-state workgroup stages in LDS.
+constexpr int kMfmaTipLdsTaxa = 128;
+
+// TL (n_taxa <= kMfmaTipLdsTaxa): the workgroup first copies the tip masks of
+// its 16 G sites into LDS (tl[16 G t + site]).
+template <typename T, int C, int V, int G, bool TL>
 __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
     const Op* __restrict__ ops, const int* __restrict__ prog_begin, const uint32_t* __restrict__ tips, T* clv,
     int* scl, const double* __restrict__ Pm, const double* __restrict__ PTm, const double* __restrict__ roots,
     const double* __restrict__ site_w, double* __restrict__ site_lnl, double* __restrict__ partial, int m, int m_pad,
-    int S, uint32_t full) {
+    int S, uint32_t full, int n_taxa) {
     __shared__ int xexp[2][C][16 * G];
     __shared__ double xroot[C][16 * G];
+    __shared__ uint32_t tl[TL ? kMfmaTipLdsTaxa * 16 * G : 1];
+    if (TL) {
+        const uint32_t* blk = tips + (size_t)blockIdx.x * 16 * G;
+        for (int i = threadIdx.x; i < n_taxa * 16 * G; i += 64 * C)
+            tl[i] = blk[(size_t)(i / (16 * G)) * m_pad + i % (16 * G)];
+        __syncthreads();
+    }
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, j = lane & 15;
     const int cat = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -580,14 +593,16 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
         for (int q = 0; q < G; ++q) sc[q] = 0;
         mfma_d4 acc[G][2];
         double val[G][V];
-        mfma_child<T, C, V, G>(op.l_kind, op.l_idx, op.l_node, cat, s, sg0, n_sg, lane, g, m_pad, S, full, tips, clv,
+        mfma_child<T, C, V, G, TL>(op.l_kind, op.l_idx, op.l_node, cat, s, sg0, n_sg, lane, g, m_pad, S, full, tips, tl,
+                                   clv,
                                slot_stride, scl, Pm, PTm, cA, sA, cB, sB, acc, sc);
 #pragma unroll
         for (int q = 0; q < G; ++q)
 #pragma unroll
             for (int t = 0; t < V; ++t) val[q][t] = acc[q][t / 4][t % 4];
         if (op.r_kind != SRC_NONE) {
-            mfma_child<T, C, V, G>(op.r_kind, op.r_idx, op.r_node, cat, s, sg0, n_sg, lane, g, m_pad, S, full, tips,
+            mfma_child<T, C, V, G, TL>(op.r_kind, op.r_idx, op.r_node, cat, s, sg0, n_sg, lane, g, m_pad, S, full,
+                                       tips, tl,
                                    clv, slot_stride, scl, Pm, PTm, cA, sA, cB, sB, acc, sc);
 #pragma unroll
             for (int q = 0; q < G; ++q)
@@ -1233,11 +1248,18 @@ constexpr int kMfmaGroups = 2;
 
 template <typename T, int C, int V>
 void launch_mfma(phylo_ctx* c, dim3 grid, double* site_lnl) {
-    hipLaunchKernelGGL((prune_mfma_kernel<T, C, V, kMfmaGroups>), grid, dim3(64 * C), 0, c->stream, c->run_ops,
-                       c->run_prog,
-                       reinterpret_cast<const uint32_t*>(c->d_tips), reinterpret_cast<T*>(c->d_clv), c->d_scl,
-                       c->d_Pm, c->d_PTm, c->d_roots, c->have_site_w ? c->d_sitew : nullptr, site_lnl, c->d_partial,
-                       c->m, c->m_pad, c->S, c->S == 32 ? 0xFFFFFFFFu : ((1u << c->S) - 1u));
+    const auto* tips = reinterpret_cast<const uint32_t*>(c->d_tips);
+    T* clv = reinterpret_cast<T*>(c->d_clv);
+    const double* sw = c->have_site_w ? c->d_sitew : nullptr;
+    const uint32_t full = c->S == 32 ? 0xFFFFFFFFu : ((1u << c->S) - 1u);
+    if (c->run_whole_tree && c->n_taxa <= kMfmaTipLdsTaxa)
+        hipLaunchKernelGGL((prune_mfma_kernel<T, C, V, kMfmaGroups, true>), grid, dim3(64 * C), 0, c->stream,
+                           c->run_ops, c->run_prog, tips, clv, c->d_scl, c->d_Pm, c->d_PTm, c->d_roots, sw, site_lnl,
+                           c->d_partial, c->m, c->m_pad, c->S, full, c->n_taxa);
+    else
+        hipLaunchKernelGGL((prune_mfma_kernel<T, C, V, kMfmaGroups, false>), grid, dim3(64 * C), 0, c->stream,
+                           c->run_ops, c->run_prog, tips, clv, c->d_scl, c->d_Pm, c->d_PTm, c->d_roots, sw, site_lnl,
+                           c->d_partial, c->m, c->m_pad, c->S, full, c->n_taxa);
 }
 
 template <typename T, int C>
