@@ -427,7 +427,13 @@ typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 
 __host__ __device__ constexpr int mfma_slots(int S) { return (S + 3) / 4; }
 
-// Pm[node][cat][T][t][lane] = P[16T + lane % 16][4t + lane / 16] (0 outside S).
+// Pm[node][cat][T][t][lane] = P[16T + lane % 16][4t + lane / 16] (0 outside S),
+// the A operand of v_mfma_f64_16x16x4f64 for row tile T and K-step t.  With
+// V = 5 (17..20 states) tile 1 holds only rows 16..19 and is instead the A
+// operand of v_mfma_f64_4x4x4f64 (four 4x4 blocks, one per 4 sites): lane l
+// = 4b + 16k + i holds P[16 + i][4t + k] for every block b.  That product
+// lands as D[lane j + 16 i] = row 16 + i of site j, the operand slot of
+// states 16..19, with no padded rows.
 __global__ void pmfma_kernel(const double* __restrict__ P, int S, int Sp, int C, int n_nodes, int V, int TN,
                              double* __restrict__ Pm) {
     const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -437,7 +443,8 @@ __global__ void pmfma_kernel(const double* __restrict__ P, int S, int Sp, int C,
     const int t = (gid / 64) % V;
     const int T = (gid / (64L * V)) % TN;
     const long mat = gid / (64L * V * TN);          // node * C + cat
-    const int row = 16 * T + l % 16, col = 4 * t + l / 16;
+    const bool short_tile = V == 5 && T == 1;
+    const int row = 16 * T + (short_tile ? l % 4 : l % 16), col = 4 * t + l / 16;
     Pm[gid] = (row < S && col < S) ? P[(mat * Sp + row) * Sp + col] : 0.0;
 }
 
@@ -530,6 +537,21 @@ __device__ inline void mfma_child(int kind, int idx, int node, int cat, const in
     for (int T2 = 0; T2 < TN; ++T2) {
 #pragma unroll
         for (int q = 0; q < G; ++q) acc[q][T2] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+        if (V == 5 && T2 == 1) {
+            // states 16..19: four 4x4x4 blocks give exactly the 4 rows
+            double d[G];
+#pragma unroll
+            for (int q = 0; q < G; ++q) d[q] = 0.0;
+#pragma unroll
+            for (int t = 0; t < V; ++t) {
+                const double a = pm[(V + t) * 64];
+#pragma unroll
+                for (int q = 0; q < G; ++q) d[q] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, x[q][t], d[q], 0, 0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < G; ++q) acc[q][1][0] = d[q];
+            continue;
+        }
 #pragma unroll
         for (int t = 0; t < V; ++t) {
             const double a = pm[(T2 * V + t) * 64];

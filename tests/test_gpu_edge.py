@@ -161,7 +161,7 @@ def test_site_weights_give_the_weighted_total(oracle):
     np.testing.assert_allclose(site, ref_site, rtol=1e-5)
 
 
-@pytest.mark.parametrize("S,C", [(5, 1), (20, 8), (32, 4)])
+@pytest.mark.parametrize("S,C", [(5, 1), (17, 2), (19, 4), (20, 8), (21, 2), (32, 4)])
 def test_wide_alphabet_limits(oracle, S, C):
     wl = synth.gtr_gamma_workload(n_tips=10, m=130, S=S, C=C, alpha=0.8, gap_frac=0.05, seed=S,
                                   rate_dist="gamma", freq_conc=2.0)
