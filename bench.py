@@ -222,7 +222,7 @@ def bench_protein(args, P, synth):
             "roofline": {"bound": "mfma", "achieved": flops * updates / ks / 1e12, "peak": FP64_MFMA_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": flops * updates / ks / 1e12 / FP64_MFMA_PEAK_TFLOPS,
                          "traffic": profiled_traffic("prune_mfma_protein20_g4_64x20k") if m == 20_000 else None,
-                         "kernel": "prune_mfma_kernel<float,4,5>", "kernel_ms": ks * 1e3,
+                         "kernel": "prune_mfma_kernel<float,4,5,2,true>", "kernel_ms": ks * 1e3,
                          "alg_flops_per_update": flops, "alg_bytes_per_update": bpu,
                          "hbm_gbs": bpu * updates / ks / 1e9, "hbm_frac": bpu * updates / ks / 1e9 / HBM_PEAK_GBS}}
 
@@ -440,7 +440,7 @@ def main():
                    "arithmetic": "f64", "lnL": lnl},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "prune4_kernel<float,4>", "kernel_ms": kernel_s * 1e3,
+                     "kernel": "prune4_kernel<float,4,true>", "kernel_ms": kernel_s * 1e3,
                      "alg_bytes_per_update": bpu},
     }
     try:

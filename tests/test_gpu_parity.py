@@ -60,6 +60,7 @@ def test_wide_state_loglik_matches_oracle(oracle, n, m, S, C, alpha, gap, seed, 
     with gpu_context(wl, precision=prec) as ctx:
         tot, site = ctx.loglik(per_site=True)
     rel = np.abs(site - ref_site) / np.abs(ref_site)
+    print(f"wide case: total diff {tot - ref_tot:+.3e}, per-site max rel {rel.max():.3e}", flush=True)
     if prec == 1:
         assert rel.max() <= 1e-9, rel.max()
     else:
