@@ -147,10 +147,11 @@ __device__ inline void block_sum_store(double v, double* dst) {
 //
 // A workgroup owns 64 consecutive sites; wave c of the workgroup computes rate
 // category c for them, so the P matrices a wave needs are uniform across it
-// (scalar loads) and each lane holds one float4 partial.  The categories of a
-// site share one power-of-two scaler: each wave proposes the frexp exponent of
-// its per-site maximum, the proposals meet in LDS (double-buffered, one
-// barrier per op) and every wave applies the largest.
+// (scalar loads) and each lane holds one float4 partial.  Each (node, site,
+// category) partial carries its own power-of-two scaler (the frexp exponent
+// of its maximum), so the waves of a workgroup walk the program without
+// waiting for each other; they meet only at the root, where the categories'
+// sums are brought to a common exponent.
 // walk4 runs ops[k0, k1) for site s and returns the site's lnL (root op).
 // With TL the tip codes of the workgroup's 64 sites come from LDS (tl: taxon
 // t's sites 2b and 2b+1 in the low and high nibble of tl[32 t + b]).
@@ -158,18 +159,12 @@ template <typename T, int C, bool TL>
 __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const uint8_t* __restrict__ tips,
                                const uint8_t* tl, T* clv, int* scl, const double* __restrict__ P,
                                const double* __restrict__ roots, int m_pad, int s, int lane, int cat,
-                               int (&xexp)[2][C][64], double (&xroot)[C][64], double2 (&xb)[C][2][64]) {
+                               int (&xsc)[C][64], double (&xroot)[C][64], double2 (&xb)[C][2][64]) {
     // carried results stay in FP64: only stored partials are rounded to T
     // (carry B, the one result waiting while a sibling subtree is walked,
     // sits in LDS: xb[cat][][lane], written and read by the same lane)
     double cA[4] = {0.0, 0.0, 0.0, 0.0};
     int sA = 0, sB = 0;
-    // the previous op's stored values: held in their registers until this op's
-    // exchange, so the registers a store reads are not reused while the store
-    // may still be reading them (that reuse makes the compiler wait for the
-    // store to complete before the exchange)
-    T held[4] = {T(0), T(0), T(0), T(0)};
-    int held_sc = 0;
     double lnl = 0.0;
     const T* plane = clv + ((size_t)cat * m_pad + s) * 4;          // + slot * C * m_pad * 4
     const size_t slot_stride = (size_t)C * m_pad * 4;
@@ -189,7 +184,7 @@ __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const
                 for (int j = 0; j < 4; ++j) x[j] = (code >> j) & 1u ? 1.0 : 0.0;
             } else if (op.l_kind == SRC_CLV) {
                 load4(plane + op.l_idx * slot_stride, x);
-                sc += scl[(size_t)op.l_idx * m_pad + s];
+                sc += scl[((size_t)op.l_idx * C + cat) * m_pad + s];
             } else if (op.l_kind == SRC_CARRY_A) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) x[j] = cA[j];
@@ -218,7 +213,7 @@ __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const
                 for (int j = 0; j < 4; ++j) x[j] = (code >> j) & 1u ? 1.0 : 0.0;
             } else if (op.r_kind == SRC_CLV) {
                 load4(plane + op.r_idx * slot_stride, x);
-                sc += scl[(size_t)op.r_idx * m_pad + s];
+                sc += scl[((size_t)op.r_idx * C + cat) * m_pad + s];
             } else if (op.r_kind == SRC_CARRY_A) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) x[j] = cA[j];
@@ -240,20 +235,10 @@ __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const
 #pragma unroll
             for (int i = 0; i < 4; ++i) L[i] *= R[i];
         }
-        // shared power-of-two scaler across the categories of a site
+        // this category's power-of-two scaler: the partial's maximum to [0.5, 1)
         const double mx = fmax(fmax(L[0], L[1]), fmax(L[2], L[3]));
-        int e = INT_MIN;
+        int e = 0;
         if (mx > 0.0) (void)frexp(mx, &e);
-        if (C > 1) {
-            const int buf = k & 1;
-            xexp[buf][cat][lane] = e;
-            __syncthreads();
-            if (sizeof(T) == 4)
-                asm volatile("" ::"v"(held[0]), "v"(held[1]), "v"(held[2]), "v"(held[3]), "v"(held_sc));
-#pragma unroll
-            for (int c = 0; c < C; ++c) e = max(e, xexp[buf][c][lane]);
-        }
-        if (e == INT_MIN) e = 0;
         sc += e;
 #pragma unroll
         for (int i = 0; i < 4; ++i) L[i] = ldexp(L[i], -e);
@@ -265,10 +250,7 @@ __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const
 #pragma unroll
             for (int i = 0; i < 4; ++i) r[i] = (T)round_cut(L[i], key + 0x9e3779b9u * i, T());
             store4(const_cast<T*>(plane) + op.out_slot * slot_stride, r);
-            if (cat == 0) scl[(size_t)op.out_slot * m_pad + s] = sc;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) held[i] = r[i];
-            held_sc = sc;
+            scl[((size_t)op.out_slot * C + cat) * m_pad + s] = sc;
         }
         if (op.flags & OP_TO_B) {
             xb[cat][0][lane] = make_double2(L[0], L[1]);
@@ -285,12 +267,18 @@ __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const
             for (int i = 0; i < 4; ++i) lc = fma(roots[i], L[i], lc);
             lc *= roots[kRootCatOff + cat];
             if (C > 1) {
+                // the categories' sums at their largest exponent
                 xroot[cat][lane] = lc;
+                xsc[cat][lane] = sc;
                 __syncthreads();
+                int M = xsc[0][lane];
+#pragma unroll
+                for (int c = 1; c < C; ++c) M = max(M, xsc[c][lane]);
                 lc = 0.0;
 #pragma unroll
-                for (int c = 0; c < C; ++c) lc += xroot[c][lane];
+                for (int c = 0; c < C; ++c) lc += ldexp(xroot[c][lane], xsc[c][lane] - M);
                 __syncthreads();
+                sc = M;
             }
             lnl = log(lc) + sc * kLn2;
         }
@@ -310,7 +298,7 @@ __global__ __launch_bounds__(64 * C) void prune4_kernel(
     T* clv, int* scl, const double* __restrict__ P, const double* __restrict__ roots,
     const double* __restrict__ site_w, double* __restrict__ site_lnl, double* __restrict__ partial,
     int m, int m_pad, int n_taxa) {
-    __shared__ int xexp[2][C][64];
+    __shared__ int xsc[C][64];
     __shared__ double xroot[C][64];
     __shared__ double2 xb[C][2][64];
     __shared__ uint8_t tl[TL ? kTipLdsTaxa * 32 : 1];
@@ -327,7 +315,7 @@ __global__ __launch_bounds__(64 * C) void prune4_kernel(
         __syncthreads();
     }
     const double lnl = walk4<T, C, TL>(ops, prog_begin[prog], prog_begin[prog + 1], tips, tl, clv, scl, P, roots,
-                                       m_pad, s, lane, cat, xexp, xroot, xb);
+                                       m_pad, s, lane, cat, xsc, xroot, xb);
     const bool live = s < m && cat == 0;
     if (live && site_lnl) site_lnl[(size_t)prog * m + s] = lnl;
     const double w = live ? (site_w ? site_w[s] : 1.0) : 0.0;
@@ -363,7 +351,7 @@ __global__ __launch_bounds__(64 * C) void batch4_kernel(const WorkItem* __restri
                                                         T* clv, int* scl, const double* __restrict__ P,
                                                         const double* __restrict__ roots,
                                                         double* __restrict__ partial, int blocks_per_item) {
-    __shared__ int xexp[2][C][64];
+    __shared__ int xsc[C][64];
     __shared__ double xroot[C][64];
     __shared__ double2 xb[C][2][64];
     const int lane = threadIdx.x & 63;
@@ -378,7 +366,7 @@ __global__ __launch_bounds__(64 * C) void batch4_kernel(const WorkItem* __restri
             if ((w.block + q) * 64 >= I.m_pad) break;
             const double lnl = walk4<T, C, false>(ops, k0, k1, tips + I.tips_off, nullptr, clv + I.clv_off,
                                                   scl + I.scl_off,
-                                           P + I.P_off, roots + I.roots_off, I.m_pad, s, lane, cat, xexp, xroot,
+                                           P + I.P_off, roots + I.roots_off, I.m_pad, s, lane, cat, xsc, xroot,
                                            xb);
             if (s < I.m && cat == 0) acc += lnl;
         }
@@ -1216,7 +1204,10 @@ int prepare(phylo_ctx* c) {
     const size_t elt = c->prec == PHYLO_PREC_FP64 ? sizeof(double) : sizeof(float);
     HIPCHK(c, ensure(c->d_clv, c->clv_bytes,
                      (size_t)std::max(c->n_slots, 1) * c->C * c->m_pad * clv_states(c) * elt));
-    HIPCHK(c, ensure(c->d_scl, c->scl_bytes, (size_t)std::max(c->n_slots, 1) * c->m_pad * sizeof(int)));
+    // scalers: [slot][cat][site] for the DNA walker, [slot][site] (shared by
+    // the categories) for the wide one
+    HIPCHK(c, ensure(c->d_scl, c->scl_bytes,
+                     (size_t)std::max(c->n_slots, 1) * c->m_pad * sizeof(int) * (uses_mfma(c) ? 1 : c->C)));
     if (c->dirty_P) {
         int rc = upload_pmatrices(c);
         if (rc) return rc;
@@ -1729,24 +1720,31 @@ int phylo_node_partials(phylo_ctx* c, int node, double* clv, int32_t* scale) {
                 for (int i = 0; i < c->S; ++i)
                     clv[((size_t)s * c->C + k) * c->S + i] =
                         buf[(((size_t)k * n_sg + s / 16) * V + i / 4) * 64 + (i % 4) * 16 + s % 16];
-    } else if (c->prec == PHYLO_PREC_FP64) {
-        std::vector<double> buf(n);
-        HIPCHK(c, hipMemcpyAsync(buf.data(), reinterpret_cast<double*>(c->d_clv) + slot * n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipMemcpyAsync(sc.data(), c->d_scl + (size_t)slot * c->m_pad, c->m_pad * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        for (int s = 0; s < c->m; ++s)
-            for (int k = 0; k < c->C; ++k)
-                for (int i = 0; i < c->S; ++i)
-                    clv[((size_t)s * c->C + k) * c->S + i] = buf[((size_t)k * c->m_pad + s) * Sp + i];
     } else {
-        std::vector<float> buf(n);
-        HIPCHK(c, hipMemcpyAsync(buf.data(), reinterpret_cast<float*>(c->d_clv) + slot * n, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipMemcpyAsync(sc.data(), c->d_scl + (size_t)slot * c->m_pad, c->m_pad * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        // DNA layout: [cat][site][4] partials, one scaler per (cat, site);
+        // returned at the site's largest scaler
+        std::vector<double> buf(n);
+        if (c->prec == PHYLO_PREC_FP64) {
+            HIPCHK(c, hipMemcpyAsync(buf.data(), reinterpret_cast<double*>(c->d_clv) + slot * n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        } else {
+            std::vector<float> fbuf(n);
+            HIPCHK(c, hipMemcpyAsync(fbuf.data(), reinterpret_cast<float*>(c->d_clv) + slot * n, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            std::copy(fbuf.begin(), fbuf.end(), buf.begin());
+        }
+        std::vector<int> csc((size_t)c->C * c->m_pad);
+        HIPCHK(c, hipMemcpyAsync(csc.data(), c->d_scl + (size_t)slot * c->C * c->m_pad, csc.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        for (int s = 0; s < c->m; ++s)
-            for (int k = 0; k < c->C; ++k)
+        for (int s = 0; s < c->m; ++s) {
+            int M = csc[s];
+            for (int k = 1; k < c->C; ++k) M = std::max(M, csc[(size_t)k * c->m_pad + s]);
+            sc[s] = M;
+            for (int k = 0; k < c->C; ++k) {
+                const int d = csc[(size_t)k * c->m_pad + s] - M;
                 for (int i = 0; i < c->S; ++i)
-                    clv[((size_t)s * c->C + k) * c->S + i] = buf[((size_t)k * c->m_pad + s) * Sp + i];
+                    clv[((size_t)s * c->C + k) * c->S + i] = std::ldexp(buf[((size_t)k * c->m_pad + s) * Sp + i], d);
+            }
+        }
     }
     if (scale)
         for (int s = 0; s < c->m; ++s) scale[s] = sc[s];
@@ -2165,7 +2163,7 @@ int phylo_batch_finalize(phylo_batch* b, int autotune) {
         I.clv_off = clv_elems;
         clv_elems += (long)(I.n_nodes - I.n_taxa) * b->C * I.m_pad * 4;
         I.scl_off = scl_elems;
-        scl_elems += (long)(I.n_nodes - I.n_taxa) * I.m_pad;
+        scl_elems += (long)(I.n_nodes - I.n_taxa) * b->C * I.m_pad;
         I.P_off = P_elems;
         P_elems += (long)I.n_nodes * b->C * 16;
         I.roots_off = roots_elems;
