@@ -503,7 +503,7 @@ __device__ inline void mfma_child(int kind, int idx, int node, int cat, const in
             const T* src = clv + (size_t)idx * slot_stride + ((size_t)cat * n_sg + sg0 + q) * V * 64 + lane;
 #pragma unroll
             for (int t = 0; t < V; ++t) x[q][t] = to_d(src[t * 64]);
-            sc[q] += scl[(size_t)idx * m_pad + s[q]];
+            sc[q] += scl[((size_t)idx * C + cat) * m_pad + s[q]];
         }
     } else if (kind == SRC_CARRY_A) {
 #pragma unroll
@@ -563,7 +563,7 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
     int* scl, const double* __restrict__ Pm, const double* __restrict__ PTm, const double* __restrict__ roots,
     const double* __restrict__ site_w, double* __restrict__ site_lnl, double* __restrict__ partial, int m, int m_pad,
     int S, uint32_t full, int n_taxa) {
-    __shared__ int xexp[2][C][16 * G];
+    __shared__ int xsc[C][16 * G];
     __shared__ double xroot[C][16 * G];
     __shared__ uint32_t tl[TL ? kMfmaTipLdsTaxa * 16 * G : 1];
     if (TL) {
@@ -626,7 +626,7 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
 #pragma unroll
             for (int t = 0; t < V; ++t) vr[q][t] = round_cut(val[q][t], key + 0x9e3779b9u * t, T());
         }
-        // per-site maximum over the lane groups, then over the category waves
+        // this category's scaler: the per-site maximum over the lane groups
         int e[G];
 #pragma unroll
         for (int q = 0; q < G; ++q) {
@@ -635,24 +635,11 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
             for (int t = 0; t < V; ++t) mx = fmax(mx, val[q][t]);
             mx = fmax(mx, __shfl_xor(mx, 16, 64));
             mx = fmax(mx, __shfl_xor(mx, 32, 64));
-            e[q] = INT_MIN;
+            e[q] = 0;
             if (mx > 0.0) (void)frexp(mx, &e[q]);
-        }
-        if (C > 1) {
-            const int buf = k & 1;
-            if (g == 0) {
-#pragma unroll
-                for (int q = 0; q < G; ++q) xexp[buf][cat][16 * q + j] = e[q];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < G; ++q)
-#pragma unroll
-                for (int c = 0; c < C; ++c) e[q] = max(e[q], xexp[buf][c][16 * q + j]);
         }
 #pragma unroll
         for (int q = 0; q < G; ++q) {
-            if (e[q] == INT_MIN) e[q] = 0;
             sc[q] += e[q];
             T r[V];                                  // stored and carried in the storage precision
 #pragma unroll
@@ -664,7 +651,7 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
                 T* dst = clv + (size_t)op.out_slot * slot_stride + ((size_t)cat * n_sg + sg0 + q) * V * 64 + lane;
 #pragma unroll
                 for (int t = 0; t < V; ++t) dst[t * 64] = r[t];
-                if (cat == 0 && g == 0) scl[(size_t)op.out_slot * m_pad + s[q]] = sc[q];
+                if (g == 0) scl[((size_t)op.out_slot * C + cat) * m_pad + s[q]] = sc[q];
             }
             if (op.flags & OP_TO_B) {
 #pragma unroll
@@ -688,16 +675,24 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
                 lc[q] *= roots[kRootCatOff + cat];
             }
             if (C > 1) {
+                // the categories' sums at their largest exponent
                 if (g == 0) {
 #pragma unroll
-                    for (int q = 0; q < G; ++q) xroot[cat][16 * q + j] = lc[q];
+                    for (int q = 0; q < G; ++q) {
+                        xroot[cat][16 * q + j] = lc[q];
+                        xsc[cat][16 * q + j] = sc[q];
+                    }
                 }
                 __syncthreads();
 #pragma unroll
                 for (int q = 0; q < G; ++q) {
+                    int M = xsc[0][16 * q + j];
+#pragma unroll
+                    for (int c = 1; c < C; ++c) M = max(M, xsc[c][16 * q + j]);
                     lc[q] = 0.0;
 #pragma unroll
-                    for (int c = 0; c < C; ++c) lc[q] += xroot[c][16 * q + j];
+                    for (int c = 0; c < C; ++c) lc[q] += ldexp(xroot[c][16 * q + j], xsc[c][16 * q + j] - M);
+                    sc[q] = M;
                 }
                 __syncthreads();
             }
@@ -1204,10 +1199,8 @@ int prepare(phylo_ctx* c) {
     const size_t elt = c->prec == PHYLO_PREC_FP64 ? sizeof(double) : sizeof(float);
     HIPCHK(c, ensure(c->d_clv, c->clv_bytes,
                      (size_t)std::max(c->n_slots, 1) * c->C * c->m_pad * clv_states(c) * elt));
-    // scalers: [slot][cat][site] for the DNA walker, [slot][site] (shared by
-    // the categories) for the wide one
-    HIPCHK(c, ensure(c->d_scl, c->scl_bytes,
-                     (size_t)std::max(c->n_slots, 1) * c->m_pad * sizeof(int) * (uses_mfma(c) ? 1 : c->C)));
+    // scalers: [slot][cat][site]
+    HIPCHK(c, ensure(c->d_scl, c->scl_bytes, (size_t)std::max(c->n_slots, 1) * c->C * c->m_pad * sizeof(int)));
     if (c->dirty_P) {
         int rc = upload_pmatrices(c);
         if (rc) return rc;
@@ -1699,55 +1692,41 @@ int phylo_node_partials(phylo_ctx* c, int node, double* clv, int32_t* scale) {
     const int slot = node - c->n_taxa;
     const int Sp = clv_states(c);
     const size_t n = (size_t)c->C * c->m_pad * Sp;
-    std::vector<int> sc(c->m_pad);
     HIPCHK(c, hipSetDevice(c->device));
-    if (uses_mfma(c)) {
-        // [cat][site group][V][64 lanes]: state i of site s is slot i / 4 of lane (i % 4) * 16 + s % 16
-        const int V = mfma_slots(c->S), n_sg = c->m_pad / 16;
-        std::vector<double> buf(n);
-        if (c->prec == PHYLO_PREC_FP64) {
-            HIPCHK(c, hipMemcpyAsync(buf.data(), reinterpret_cast<double*>(c->d_clv) + slot * n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        } else {
-            std::vector<float> fbuf(n);
-            HIPCHK(c, hipMemcpyAsync(fbuf.data(), reinterpret_cast<float*>(c->d_clv) + slot * n, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            std::copy(fbuf.begin(), fbuf.end(), buf.begin());
-        }
-        HIPCHK(c, hipMemcpyAsync(sc.data(), c->d_scl + (size_t)slot * c->m_pad, c->m_pad * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        for (int s = 0; s < c->m; ++s)
-            for (int k = 0; k < c->C; ++k)
-                for (int i = 0; i < c->S; ++i)
-                    clv[((size_t)s * c->C + k) * c->S + i] =
-                        buf[(((size_t)k * n_sg + s / 16) * V + i / 4) * 64 + (i % 4) * 16 + s % 16];
+    std::vector<double> buf(n);
+    if (c->prec == PHYLO_PREC_FP64) {
+        HIPCHK(c, hipMemcpyAsync(buf.data(), reinterpret_cast<double*>(c->d_clv) + slot * n, n * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
     } else {
-        // DNA layout: [cat][site][4] partials, one scaler per (cat, site);
-        // returned at the site's largest scaler
-        std::vector<double> buf(n);
-        if (c->prec == PHYLO_PREC_FP64) {
-            HIPCHK(c, hipMemcpyAsync(buf.data(), reinterpret_cast<double*>(c->d_clv) + slot * n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        } else {
-            std::vector<float> fbuf(n);
-            HIPCHK(c, hipMemcpyAsync(fbuf.data(), reinterpret_cast<float*>(c->d_clv) + slot * n, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            std::copy(fbuf.begin(), fbuf.end(), buf.begin());
-        }
-        std::vector<int> csc((size_t)c->C * c->m_pad);
-        HIPCHK(c, hipMemcpyAsync(csc.data(), c->d_scl + (size_t)slot * c->C * c->m_pad, csc.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        std::vector<float> fbuf(n);
+        HIPCHK(c, hipMemcpyAsync(fbuf.data(), reinterpret_cast<float*>(c->d_clv) + slot * n, n * sizeof(float),
+                                 hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        for (int s = 0; s < c->m; ++s) {
-            int M = csc[s];
-            for (int k = 1; k < c->C; ++k) M = std::max(M, csc[(size_t)k * c->m_pad + s]);
-            sc[s] = M;
-            for (int k = 0; k < c->C; ++k) {
-                const int d = csc[(size_t)k * c->m_pad + s] - M;
-                for (int i = 0; i < c->S; ++i)
-                    clv[((size_t)s * c->C + k) * c->S + i] = std::ldexp(buf[((size_t)k * c->m_pad + s) * Sp + i], d);
-            }
+        std::copy(fbuf.begin(), fbuf.end(), buf.begin());
+    }
+    // one scaler per (category, site): returned at the site's largest, the
+    // other categories' values shifted to it
+    std::vector<int> csc((size_t)c->C * c->m_pad);
+    HIPCHK(c, hipMemcpyAsync(csc.data(), c->d_scl + (size_t)slot * c->C * c->m_pad, csc.size() * sizeof(int),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const bool mfma = uses_mfma(c);
+    const int V = mfma_slots(c->S), n_sg = c->m_pad / 16;
+    // DNA: [cat][site][4]; wide: [cat][site group][V][64 lanes], state i of
+    // site s in slot i / 4 of lane (i % 4) * 16 + s % 16
+    const auto at = [&](int k, int s, int i) -> double {
+        return mfma ? buf[(((size_t)k * n_sg + s / 16) * V + i / 4) * 64 + (i % 4) * 16 + s % 16]
+                    : buf[((size_t)k * c->m_pad + s) * Sp + i];
+    };
+    for (int s = 0; s < c->m; ++s) {
+        int M = csc[s];
+        for (int k = 1; k < c->C; ++k) M = std::max(M, csc[(size_t)k * c->m_pad + s]);
+        if (scale) scale[s] = M;
+        for (int k = 0; k < c->C; ++k) {
+            const int d = csc[(size_t)k * c->m_pad + s] - M;
+            for (int i = 0; i < c->S; ++i) clv[((size_t)s * c->C + k) * c->S + i] = std::ldexp(at(k, s, i), d);
         }
     }
-    if (scale)
-        for (int s = 0; s < c->m; ++s) scale[s] = sc[s];
     return PHYLO_OK;
 }
 
