@@ -170,8 +170,14 @@ __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const
     const size_t slot_stride = (size_t)C * m_pad * 4;
     for (int k = k0; k < k1; ++k) {
         const Op op = ops[k];
-        const double* Pl = P + ((size_t)op.l_node * C + cat) * 16;
-        const double* Pr = P + ((size_t)op.r_node * C + cat) * 16;
+        // both children's P up front (64 SGPRs): their scalar loads overlap
+        double Pl[16], Pr[16];
+        {
+            const double* pl = P + ((size_t)op.l_node * C + cat) * 16;
+            const double* pr = P + ((size_t)op.r_node * C + cat) * 16;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) { Pl[i] = pl[i]; Pr[i] = pr[i]; }
+        }
         // child inputs
         double L[4], R[4];
         int sc = 0;
