@@ -441,7 +441,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "prune4_kernel<float,4,true>", "kernel_ms": kernel_s * 1e3,
-                     "alg_bytes_per_update": bpu},
+                     "alg_bytes_per_update": bpu,
+                     # the algorithmic bytes assume a CLV store per node: every internal
+                     # partial written once and every stored child read back.  The kernel
+                     # keeps most partials in registers / LDS, so it moves far fewer bytes
+                     # (traffic) and frac can pass 1; traffic_gbs is what HBM really serves.
+                     "traffic_gbs": traffic / kernel_s / 1e9 if traffic else None},
     }
     try:
         cgbs = copy_bandwidth_subprocess()
