@@ -129,6 +129,31 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
+// Compensated sum of partial[0, n) by one wave in a fixed order: lane l adds
+// partial[l], partial[l + 64], ... with Kahan compensation, then the lanes'
+// (sum, error) pairs meet in a two-sum butterfly.  Every lane returns the
+// same total (two-sum gives the exact error whichever operand comes first).
+__device__ inline double wave_total(const double* partial, int n, int lane) {
+    double s = 0.0, comp = 0.0;
+    for (int i = lane; i < n; i += 64) {
+        const double x = partial[i] - comp;
+        const double t = s + x;
+        comp = (t - s) - x;
+        s = t;
+    }
+    double hi = s, lo = -comp;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double b = __shfl_xor(hi, o, 64), bl = __shfl_xor(lo, o, 64);
+        const double t = hi + b;
+        const double bb = t - hi;
+        const double err = (hi - (t - bb)) + (b - bb);
+        hi = t;
+        lo = (lo + bl) + err;
+    }
+    return hi + lo;
+}
+
 // Sums v over the workgroup (any multiple of 64 threads); thread 0 stores it.
 __device__ inline void block_sum_store(double v, double* dst) {
     __shared__ double red[16];
@@ -717,33 +742,11 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
 }
 
 // Compensated fixed-order sum of the per-workgroup partials of each program
-// (one 256-thread block per program: strided Kahan sums, then a fixed tree).
-__global__ __launch_bounds__(256) void final_sum_kernel(const double* __restrict__ partial, int n_part,
-                                                       double* __restrict__ out) {
-    __shared__ double hi[256], lo[256];
-    const int y = blockIdx.x;
-    double s = 0.0, comp = 0.0;
-    for (int i = threadIdx.x; i < n_part; i += 256) {
-        const double x = partial[(size_t)y * n_part + i] - comp;
-        const double t = s + x;
-        comp = (t - s) - x;
-        s = t;
-    }
-    hi[threadIdx.x] = s;
-    lo[threadIdx.x] = -comp;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            const double a = hi[threadIdx.x], b = hi[threadIdx.x + w];
-            const double t = a + b;
-            const double bb = t - a;
-            const double err = (a - (t - bb)) + (b - bb);   // two-sum
-            hi[threadIdx.x] = t;
-            lo[threadIdx.x] += lo[threadIdx.x + w] + err;
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[y] = hi[0] + lo[0];
+// (one wave per program, wave_total).
+__global__ __launch_bounds__(64) void final_sum_kernel(const double* __restrict__ partial, int n_part,
+                                                      double* __restrict__ out) {
+    const double t = wave_total(partial + (size_t)blockIdx.x * n_part, n_part, threadIdx.x);
+    if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
 
 // P[node][cat][i][j] = sum_k V[i][k] exp(lam_k r_c t_node) Vinv[k][j], padded to Sp x Sp with zeros.
@@ -880,10 +883,10 @@ struct phylo_ctx {
     double* d_sitew = nullptr; size_t sitew_bytes = 0;
     double* d_sitelnl = nullptr; size_t sitelnl_bytes = 0;
     double* d_partial = nullptr; size_t partial_bytes = 0;
-    double* d_total = nullptr; size_t total_bytes = 0;
     double* d_eig = nullptr; size_t eig_bytes = 0;
     double* d_eff = nullptr; size_t eff_bytes = 0;
-    double* h_total = nullptr; size_t h_total_bytes = 0;  // pinned
+    double* h_total = nullptr; size_t h_total_bytes = 0;  // pinned, mapped: final_sum_kernel writes here
+    double* h_total_dev = nullptr;                        // its device address
     // Pinned staging for the per-evaluation uploads (eigen system + edge
     // lengths, root / category weights): copied without a stream
     // synchronisation; an event guards reuse of the host buffer.
@@ -1308,12 +1311,13 @@ int run_programs(phylo_ctx* c, const Op* d_ops, const int* d_prog, int n_prog, d
     const bool mfma = uses_mfma(c);
     const int nblk = c->m_pad / (mfma ? 16 * kMfmaGroups : 64);
     HIPCHK(c, ensure(c->d_partial, c->partial_bytes, (size_t)n_prog * nblk * sizeof(double)));
-    HIPCHK(c, ensure(c->d_total, c->total_bytes, (size_t)n_prog * sizeof(double)));
     if (site_lnl_host) HIPCHK(c, ensure(c->d_sitelnl, c->sitelnl_bytes, (size_t)n_prog * c->m * sizeof(double)));
     if (c->h_total_bytes < (size_t)n_prog * sizeof(double)) {
         if (c->h_total) HIPCHK(c, hipHostFree(c->h_total));
         c->h_total = nullptr;
-        HIPCHK(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_total), (size_t)n_prog * sizeof(double)));
+        HIPCHK(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_total), (size_t)n_prog * sizeof(double),
+                                hipHostMallocMapped));
+        HIPCHK(c, hipHostGetDevicePointer(reinterpret_cast<void**>(&c->h_total_dev), c->h_total, 0));
         c->h_total_bytes = (size_t)n_prog * sizeof(double);
     }
     dim3 grid(nblk, n_prog);
@@ -1327,9 +1331,9 @@ int run_programs(phylo_ctx* c, const Op* d_ops, const int* d_prog, int n_prog, d
     if (rc) return rc;
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    hipLaunchKernelGGL(final_sum_kernel, dim3(n_prog), dim3(256), 0, c->stream, c->d_partial, nblk, c->d_total);
+    // the totals go straight into mapped pinned memory: no copy
+    hipLaunchKernelGGL(final_sum_kernel, dim3(n_prog), dim3(64), 0, c->stream, c->d_partial, nblk, c->h_total_dev);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(c->h_total, c->d_total, (size_t)n_prog * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     if (site_lnl_host)
         HIPCHK(c, hipMemcpyAsync(site_lnl_host, c->d_sitelnl, (size_t)n_prog * c->m * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1504,7 +1508,7 @@ void phylo_destroy(phylo_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* bufs[] = {c->d_tips, c->d_Pm, c->d_PTm, c->d_clv, c->d_scl, c->d_P, c->d_ops, c->d_prog, c->d_nni_ops, c->d_nni_prog,
                     c->d_roots, c->d_sitew,
-                    c->d_sitelnl, c->d_partial, c->d_total, c->d_eig, c->d_eff};
+                    c->d_sitelnl, c->d_partial, c->d_eig, c->d_eff};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->h_total) (void)hipHostFree(c->h_total);
