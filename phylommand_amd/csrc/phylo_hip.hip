@@ -2,15 +2,15 @@
 //
 // Design
 //  * A likelihood evaluation is compiled on the host into a "pruning
-//    program": the internal nodes in the reference's postorder (left subtree,
-//    right subtree, node; simpleML.cpp:127-184), each op naming where its two
-//    child partials come from (a tip row, a stored partial, or a register
-//    carry holding the result of an earlier op of the same program).
+//    program": every internal node once, children before parents (the
+//    reference recurses left subtree, right subtree, node;
+//    simpleML.cpp:127-184), each op naming where its two child partials come
+//    from (a tip row, a stored partial, or one of two carries holding the
+//    result of an earlier op of the same program).
 //  * One launch walks the whole program.  Each work-item owns one site and
 //    runs every op for it, so no inter-workgroup dependency exists and all
-//    postorder levels are fused into the launch.  The partial of the op just
-//    computed stays in registers; when the next op consumes it, it is not
-//    read back.
+//    postorder levels are fused into the launch.  Only partials the walk
+//    reads back from memory are stored at all.
 //  * Partials are stored site-major and state-interleaved:
 //    clv[slot][cat][site][4] (one 16-byte float4 per work-item per category),
 //    so every global load/store of a wavefront is one contiguous 1 KiB burst.
@@ -20,10 +20,10 @@
 //    the compiler keeps them in scalar registers.
 //  * Every node's partial is renormalised by an exact power of two (frexp /
 //    ldexp) so its largest entry is in [0.5, 1); the integer exponent is
-//    accumulated into one scaler per (node, site) shared by all categories.
-//  * Per-site lnL = log(sum_c w_c sum_i rw_i L_ci) + scaler*ln2 in FP64,
+//    accumulated into one scaler per (node, category, site).
+//  * Per-site lnL = log(sum_c w_c sum_i rw_i L_ci 2^scaler_c) in FP64,
 //    reduced per workgroup with wavefront shuffles, then summed in a fixed
-//    order with compensation by a second tiny launch.
+//    order with compensation by a second one-wave launch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -424,8 +424,8 @@ __global__ __launch_bounds__(64) void segment_sum_kernel(const double* __restric
 // ---------------------------------------------------------------------------
 // Wide-state walker on the matrix cores (4 < S <= 32, FP64 arithmetic).
 //
-// One wave = 16 sites x one rate category; a workgroup holds the C category
-// waves of one 16-site group.  Each child product is a GEMM
+// One wave = G groups of 16 sites x one rate category; a workgroup holds the
+// C category waves of the same G site groups.  Each child product is a GEMM
 //     out[state row][site] = sum_k P[row][k] * L[k][site]
 // done as v_mfma_f64_16x16x4f64 steps (A = P: 16 rows x 4 k, B = L: 4 k x 16
 // sites, D = 16 rows x 16 sites; D[i][j] sits in lane j + 16 (i % 4),
@@ -433,13 +433,14 @@ __global__ __launch_bounds__(64) void segment_sum_kernel(const double* __restric
 // t of a lane holds state 4t + g, and row i of tile T is state 16T + i, so an
 // MFMA result (tile T, register r) is slot 4T + r of the next op's operand
 // with no shuffles.  V = ceil(S / 4) slots cover the S states; tile 1 exists
-// for S > 16.  Partials are stored as [slot][cat][site group][V][64 lanes]
+// for S > 16 (for 17..20 states as v_mfma_f64_4x4x4f64 blocks, see
+// pmfma_kernel).  Partials are stored as [slot][cat][site group][V][64 lanes]
 // (every load and store a full row), P as [node][cat][tile][V][64] doubles
 // (pmfma_kernel).  FP64 matrix cores rather than FP32 ones: the FP32 MFMA
 // dot products truncate small addends (about -0.1 ulp per 5-step chain on
 // transition-matrix-shaped operands), which biases every site the same way.
-// Scalers as in the other walkers: one int32 power of two per (node, site)
-// shared by the categories, exchanged through LDS with one barrier per op.
+// Scalers as in the DNA walker: one int32 power of two per (node, category,
+// site), the categories meeting only at the root.
 // ---------------------------------------------------------------------------
 
 typedef double mfma_d4 __attribute__((ext_vector_type(4)));
