@@ -180,9 +180,15 @@ int phylo_batch_add(phylo_batch *b, int n_taxa, int n_sites, int n_states, const
                     const double *root_weights, const double *pi, const double *cat_rates,
                     const double *cat_weights, int nni_mode, int *index);
 
+/* The tiling finalize uses without autotuning: sites per work item (a
+ * multiple of 64, at most 4096; default 128) and rate-category waves per
+ * workgroup (n_cat or 1; default n_cat).  Call before phylo_batch_finalize. */
+int phylo_batch_set_tiling(phylo_batch *b, int sites_per_item, int category_waves);
+
 /* Uploads everything, builds the work lists and captures the graph.  With
- * autotune != 0 it first times 64, 128, 256 and 512 sites per work item and
- * keeps the fastest. */
+ * autotune != 0 it first times 64, 128, 256 and 512 sites per work item with
+ * one rate category per wave, then one wave walking every category at the
+ * best size and twice it, and keeps the fastest (phylo_batch_tiling). */
 int phylo_batch_finalize(phylo_batch *b, int autotune);
 
 /* Replays the graph.  totals: one lnL per problem; nni_lnl (nullable): every
@@ -196,6 +202,11 @@ int phylo_batch_nni_layout(phylo_batch *b, int32_t *offsets, int32_t *node, int3
  * last run (ms), the chosen sites per work item and the variant count. */
 int phylo_batch_stats(phylo_batch *b, double *updates_full, double *updates_nni, double *last_ms,
                       int *sites_per_item, int *n_variants);
+
+/* The tiling the autotuner chose: sites per work item and rate-category
+ * waves per workgroup (n_cat: one category per wave; 1: one wave walks every
+ * category in turn). */
+int phylo_batch_tiling(phylo_batch *b, int *sites_per_item, int *category_waves);
 
 void phylo_batch_destroy(phylo_batch *b);
 

@@ -58,6 +58,8 @@ SIGNATURES = [
     ("phylo_batch_nni_layout", _c_int, [_c_void_p, _c_int_p, _c_int_p, _c_int_p]),
     ("phylo_batch_stats", _c_int, [_c_void_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.POINTER(ctypes.c_int),
                                    ctypes.POINTER(ctypes.c_int)]),
+    ("phylo_batch_set_tiling", _c_int, [_c_void_p, _c_int, _c_int]),
+    ("phylo_batch_tiling", _c_int, [_c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     ("phylo_batch_destroy", None, [_c_void_p]),
     ("phylo_last_kernel_ms", _c_int, [_c_void_p, _c_dbl_p]),
     ("phylo_simulate", _c_int, [_c_void_p, _c_int, _c_dbl_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint8)]),
@@ -392,6 +394,16 @@ class Batch:
         spi, nv = ctypes.c_int(0), ctypes.c_int(0)
         self._check(self.lib.phylo_batch_stats(self.h, _dp(u1), _dp(u2), _dp(ms), ctypes.byref(spi), ctypes.byref(nv)))
         return float(u1[0]), float(u2[0]), float(ms[0]), spi.value, nv.value
+
+    def set_tiling(self, sites_per_item: int, category_waves: int):
+        """Tiling used by finalize(autotune=False)."""
+        self._check(self.lib.phylo_batch_set_tiling(self.h, int(sites_per_item), int(category_waves)))
+
+    def tiling(self):
+        """(sites per work item, category waves per workgroup) chosen by finalize."""
+        spi, cw = ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self.lib.phylo_batch_tiling(self.h, ctypes.byref(spi), ctypes.byref(cw)))
+        return spi.value, cw.value
 
     def close(self):
         if getattr(self, "h", None):

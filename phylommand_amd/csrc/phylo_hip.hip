@@ -177,20 +177,23 @@ __device__ inline void block_sum_store(double v, double* dst) {
 // of its maximum), so the waves of a workgroup walk the program without
 // waiting for each other; they meet only at the root, where the categories'
 // sums are brought to a common exponent.
-// walk4 runs ops[k0, k1) for site s and returns the site's lnL (root op).
+// walk4 runs ops[k0, k1) for site s and category cat; at the root op it sets
+// root_lc = w_cat sum_i rw_i L_i and root_sc, the category's scaler, so that
+// its share of the site likelihood is root_lc 2^root_sc.
 // With TL the tip codes of the workgroup's 64 sites come from LDS (tl: taxon
 // t's sites 2b and 2b+1 in the low and high nibble of tl[32 t + b]).
 template <typename T, int C, bool TL>
-__device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const uint8_t* __restrict__ tips,
-                               const uint8_t* tl, T* clv, int* scl, const double* __restrict__ P,
-                               const double* __restrict__ roots, int m_pad, int s, int lane, int cat,
-                               int (&xsc)[C][64], double (&xroot)[C][64], double2 (&xb)[C][2][64]) {
+__device__ inline void walk4(const Op* __restrict__ ops, int k0, int k1, const uint8_t* __restrict__ tips,
+                             const uint8_t* tl, T* clv, int* scl, const double* __restrict__ P,
+                             const double* __restrict__ roots, int m_pad, int s, int lane, int cat,
+                             double2 (&xb)[C][2][64], double& root_lc, int& root_sc) {
     // carried results stay in FP64: only stored partials are rounded to T
     // (carry B, the one result waiting while a sibling subtree is walked,
     // sits in LDS: xb[cat][][lane], written and read by the same lane)
     double cA[4] = {0.0, 0.0, 0.0, 0.0};
     int sA = 0, sB = 0;
-    double lnl = 0.0;
+    root_lc = 0.0;
+    root_sc = 0;
     const T* plane = clv + ((size_t)cat * m_pad + s) * 4;          // + slot * C * m_pad * 4
     const size_t slot_stride = (size_t)C * m_pad * 4;
     for (int k = k0; k < k1; ++k) {
@@ -296,25 +299,32 @@ __device__ inline double walk4(const Op* __restrict__ ops, int k0, int k1, const
             double lc = 0.0;
 #pragma unroll
             for (int i = 0; i < 4; ++i) lc = fma(roots[i], L[i], lc);
-            lc *= roots[kRootCatOff + cat];
-            if (C > 1) {
-                // the categories' sums at their largest exponent
-                xroot[cat][lane] = lc;
-                xsc[cat][lane] = sc;
-                __syncthreads();
-                int M = xsc[0][lane];
-#pragma unroll
-                for (int c = 1; c < C; ++c) M = max(M, xsc[c][lane]);
-                lc = 0.0;
-#pragma unroll
-                for (int c = 0; c < C; ++c) lc += ldexp(xroot[c][lane], xsc[c][lane] - M);
-                __syncthreads();
-                sc = M;
-            }
-            lnl = log(lc) + sc * kLn2;
+            root_lc = lc * roots[kRootCatOff + cat];
+            root_sc = sc;
         }
     }
-    return lnl;
+}
+
+// Site lnL from the categories' root sums (lc, sc) held by the C waves of a
+// workgroup (wave = category): the sums meet in LDS at their largest
+// exponent.  Every wave returns the same value.
+template <int C>
+__device__ inline double combine_categories(double lc, int sc, int cat, int lane, double (&xroot)[C][64],
+                                            int (&xsc)[C][64]) {
+    if (C > 1) {
+        xroot[cat][lane] = lc;
+        xsc[cat][lane] = sc;
+        __syncthreads();
+        int M = xsc[0][lane];
+#pragma unroll
+        for (int c = 1; c < C; ++c) M = max(M, xsc[c][lane]);
+        lc = 0.0;
+#pragma unroll
+        for (int c = 0; c < C; ++c) lc += ldexp(xroot[c][lane], xsc[c][lane] - M);
+        __syncthreads();
+        sc = M;
+    }
+    return log(lc) + sc * kLn2;
 }
 
 // Taxa whose tip codes a DNA workgroup stages in LDS (4 bits per site).
@@ -345,8 +355,11 @@ __global__ __launch_bounds__(64 * C) void prune4_kernel(
         }
         __syncthreads();
     }
-    const double lnl = walk4<T, C, TL>(ops, prog_begin[prog], prog_begin[prog + 1], tips, tl, clv, scl, P, roots,
-                                       m_pad, s, lane, cat, xsc, xroot, xb);
+    double lc;
+    int sc;
+    walk4<T, C, TL>(ops, prog_begin[prog], prog_begin[prog + 1], tips, tl, clv, scl, P, roots, m_pad, s, lane, cat, xb,
+                    lc, sc);
+    const double lnl = combine_categories<C>(lc, sc, cat, lane, xroot, xsc);
     const bool live = s < m && cat == 0;
     if (live && site_lnl) site_lnl[(size_t)prog * m + s] = lnl;
     const double w = live ? (site_w ? site_w[s] : 1.0) : 0.0;
@@ -374,19 +387,24 @@ struct WorkItem {
     int inst, prog, block, out;
 };
 
-template <typename T, int C>
-__global__ __launch_bounds__(64 * C) void batch4_kernel(const WorkItem* __restrict__ items, int n_items,
-                                                        const BatchInst* __restrict__ inst,
-                                                        const int* __restrict__ prog_begin,
-                                                        const Op* __restrict__ ops, const uint8_t* __restrict__ tips,
-                                                        T* clv, int* scl, const double* __restrict__ P,
-                                                        const double* __restrict__ roots,
-                                                        double* __restrict__ partial, int blocks_per_item) {
+// Category tiling CW: CW = C puts the C categories on the C waves of a
+// workgroup (they meet in LDS at the root); CW = 1 has one wave walk the
+// program once per category and combine the root sums in registers, with no
+// barrier at all (a better fit for short programs such as NNI variants).
+template <typename T, int C, int CW>
+__global__ __launch_bounds__(64 * CW) void batch4_kernel(const WorkItem* __restrict__ items, int n_items,
+                                                         const BatchInst* __restrict__ inst,
+                                                         const int* __restrict__ prog_begin,
+                                                         const Op* __restrict__ ops, const uint8_t* __restrict__ tips,
+                                                         T* clv, int* scl, const double* __restrict__ P,
+                                                         const double* __restrict__ roots,
+                                                         double* __restrict__ partial, int blocks_per_item) {
+    static_assert(CW == C || CW == 1, "categories per workgroup: all on separate waves, or one wave");
     __shared__ int xsc[C][64];
     __shared__ double xroot[C][64];
     __shared__ double2 xb[C][2][64];
     const int lane = threadIdx.x & 63;
-    const int cat = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wcat = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
         const WorkItem w = items[it];
         const BatchInst I = inst[w.inst];
@@ -395,11 +413,29 @@ __global__ __launch_bounds__(64 * C) void batch4_kernel(const WorkItem* __restri
         for (int q = 0; q < blocks_per_item; ++q) {
             const int s = (w.block + q) * 64 + lane;
             if ((w.block + q) * 64 >= I.m_pad) break;
-            const double lnl = walk4<T, C, false>(ops, k0, k1, tips + I.tips_off, nullptr, clv + I.clv_off,
-                                                  scl + I.scl_off,
-                                           P + I.P_off, roots + I.roots_off, I.m_pad, s, lane, cat, xsc, xroot,
-                                           xb);
-            if (s < I.m && cat == 0) acc += lnl;
+            double lnl;
+            if (CW == C) {
+                double lc;
+                int sc;
+                walk4<T, C, false>(ops, k0, k1, tips + I.tips_off, nullptr, clv + I.clv_off, scl + I.scl_off,
+                                   P + I.P_off, roots + I.roots_off, I.m_pad, s, lane, wcat, xb, lc, sc);
+                lnl = combine_categories<C>(lc, sc, wcat, lane, xroot, xsc);
+            } else {
+                double lc[C];
+                int sc[C];
+                for (int cat = 0; cat < C; ++cat)
+                    walk4<T, C, false>(ops, k0, k1, tips + I.tips_off, nullptr, clv + I.clv_off, scl + I.scl_off,
+                                       P + I.P_off, roots + I.roots_off, I.m_pad, s, lane, cat, xb, lc[cat],
+                                       sc[cat]);
+                int M = sc[0];
+#pragma unroll
+                for (int c = 1; c < C; ++c) M = max(M, sc[c]);
+                double t = 0.0;
+#pragma unroll
+                for (int c = 0; c < C; ++c) t += ldexp(lc[c], sc[c] - M);
+                lnl = log(t) + M * kLn2;
+            }
+            if (s < I.m && wcat == 0) acc += lnl;
         }
         block_sum_store(acc, partial + w.out);
     }
@@ -1905,7 +1941,8 @@ struct phylo_batch {
     std::string err;
     std::vector<Inst> inst;
     bool finalized = false;
-    int blocks_per_item = 1, grid = 0, n_items1 = 0, n_items2 = 0, n_seg1 = 0, n_seg2 = 0, n_var = 0;
+    int cat_waves = 4;                // category tiling: C waves per workgroup, or 1
+    int blocks_per_item = 2, grid = 0, n_items1 = 0, n_items2 = 0, n_seg1 = 0, n_seg2 = 0, n_var = 0;
     double updates1 = 0, updates2 = 0, last_ms = 0;
     std::vector<int> var_offsets;     // per instance into the flat variant list
     // device
@@ -1941,8 +1978,37 @@ int fail(phylo_batch* b, int code, const std::string& msg) {
 
 template <typename T, int C>
 void launch_batch(phylo_batch* b, const WorkItem* items, int n_items, int bpi) {
-    hipLaunchKernelGGL((batch4_kernel<T, C>), dim3(b->grid), dim3(64 * C), 0, b->stream, items, n_items, b->d_inst, b->d_prog, b->d_ops, b->d_tips, reinterpret_cast<T*>(b->d_clv), b->d_scl, b->d_P,
-                       b->d_roots, b->d_partial, bpi);
+    if (b->cat_waves == 1)
+        hipLaunchKernelGGL((batch4_kernel<T, C, 1>), dim3(b->grid), dim3(64), 0, b->stream, items, n_items,
+                           b->d_inst, b->d_prog, b->d_ops, b->d_tips, reinterpret_cast<T*>(b->d_clv), b->d_scl,
+                           b->d_P, b->d_roots, b->d_partial, bpi);
+    else
+        hipLaunchKernelGGL((batch4_kernel<T, C, C>), dim3(b->grid), dim3(64 * C), 0, b->stream, items, n_items,
+                           b->d_inst, b->d_prog, b->d_ops, b->d_tips, reinterpret_cast<T*>(b->d_clv), b->d_scl,
+                           b->d_P, b->d_roots, b->d_partial, bpi);
+}
+
+// The persistent grid of the current category tiling: as many workgroups as
+// can be resident at once.
+int batch_grid(phylo_batch* b) {
+    hipDeviceProp_t prop;
+    HIPCHK(b, hipGetDeviceProperties(&prop, b->device));
+    const bool f64 = b->prec == PHYLO_PREC_FP64, one = b->cat_waves == 1;
+    const void* kfn = nullptr;
+#define PHYLO_BATCH_KFN(CC)                                                                                   \
+    kfn = f64 ? (one ? (const void*)batch4_kernel<double, CC, 1> : (const void*)batch4_kernel<double, CC, CC>) \
+              : (one ? (const void*)batch4_kernel<float, CC, 1> : (const void*)batch4_kernel<float, CC, CC>)
+    switch (b->C) {
+    case 1: PHYLO_BATCH_KFN(1); break;
+    case 2: PHYLO_BATCH_KFN(2); break;
+    case 4: PHYLO_BATCH_KFN(4); break;
+    default: PHYLO_BATCH_KFN(8); break;
+    }
+#undef PHYLO_BATCH_KFN
+    int per_cu = 1;
+    HIPCHK(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 64 * b->cat_waves, 0));
+    b->grid = std::max(1, per_cu) * prop.multiProcessorCount;
+    return PHYLO_OK;
 }
 
 int enqueue_batch_phase(phylo_batch* b, int phase, int bpi) {
@@ -2057,6 +2123,7 @@ int phylo_batch_create(int device, int n_cat, int precision, phylo_batch** out) 
     phylo_batch* b = new phylo_batch();
     b->device = device;
     b->C = n_cat;
+    b->cat_waves = n_cat;
     b->prec = precision;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
@@ -2226,30 +2293,26 @@ int phylo_batch_finalize(phylo_batch* b, int autotune) {
     const int n_out = (int)b->inst.size() + b->n_var;
     HIPCHK(b, hipMalloc(&b->d_out, std::max(n_out, 1) * sizeof(double)));
     HIPCHK(b, hipHostMalloc(reinterpret_cast<void**>(&b->h_out), std::max(n_out, 1) * sizeof(double)));
-    // persistent grid: as many workgroups as can be resident at once
-    hipDeviceProp_t prop;
-    HIPCHK(b, hipGetDeviceProperties(&prop, b->device));
-    int per_cu = 1;
-    const int threads = 64 * b->C;
-    const bool f64 = b->prec == PHYLO_PREC_FP64;
-    const void* kfn = nullptr;
-    switch (b->C) {
-    case 1: kfn = f64 ? (const void*)batch4_kernel<double, 1> : (const void*)batch4_kernel<float, 1>; break;
-    case 2: kfn = f64 ? (const void*)batch4_kernel<double, 2> : (const void*)batch4_kernel<float, 2>; break;
-    case 4: kfn = f64 ? (const void*)batch4_kernel<double, 4> : (const void*)batch4_kernel<float, 4>; break;
-    default: kfn = f64 ? (const void*)batch4_kernel<double, 8> : (const void*)batch4_kernel<float, 8>; break;
+    // autotune the work-item size (64-site blocks per item) with the
+    // categories on separate waves, then the one-wave category tiling around
+    // the best size
+    struct Cfg { int cw, bpi; };
+    std::vector<Cfg> cands;
+    if (autotune) {
+        for (int bpi : {1, 2, 4, 8}) cands.push_back({b->C, bpi});
+    } else {
+        cands.push_back({b->cat_waves, b->blocks_per_item});
     }
-    HIPCHK(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, threads, 0));
-    b->grid = std::max(1, per_cu) * prop.multiProcessorCount;
-    // autotune the number of 64-site blocks per work item
-    std::vector<int> cands = autotune ? std::vector<int>{1, 2, 4, 8} : std::vector<int>{2};
     float best = 1e30f;
-    int best_bpi = cands.front();
-    for (int bpi : cands) {
-        build_work_lists(b, bpi, prog_len);
-        int rc = upload_work_lists(b);
+    Cfg best_cfg = cands.front();
+    auto trial = [&](const Cfg& cfg) -> int {
+        b->cat_waves = cfg.cw;
+        int rc = batch_grid(b);
         if (rc) return rc;
-        b->blocks_per_item = bpi;
+        build_work_lists(b, cfg.bpi, prog_len);
+        rc = upload_work_lists(b);
+        if (rc) return rc;
+        b->blocks_per_item = cfg.bpi;
         float ms = 0.f;
         for (int rep = 0; rep < (autotune ? 2 : 0); ++rep) {
             HIPCHK(b, hipEventRecord(b->ev0, b->stream));
@@ -2259,13 +2322,28 @@ int phylo_batch_finalize(phylo_batch* b, int autotune) {
             HIPCHK(b, hipStreamSynchronize(b->stream));
             HIPCHK(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
         }
-        if (autotune && ms < best) { best = ms; best_bpi = bpi; }
-    }
-    if (b->blocks_per_item != best_bpi) {
-        build_work_lists(b, best_bpi, prog_len);
-        int rc = upload_work_lists(b);
+        if (autotune && ms < best) { best = ms; best_cfg = cfg; }
+        return PHYLO_OK;
+    };
+    for (const Cfg& cfg : cands) {
+        int rc = trial(cfg);
         if (rc) return rc;
-        b->blocks_per_item = best_bpi;
+    }
+    if (autotune && b->C > 1) {
+        const int bpi0 = best_cfg.bpi;
+        for (int bpi : {bpi0, 2 * bpi0}) {
+            int rc = trial({1, bpi});
+            if (rc) return rc;
+        }
+    }
+    if (b->cat_waves != best_cfg.cw || b->blocks_per_item != best_cfg.bpi) {
+        b->cat_waves = best_cfg.cw;
+        int rc = batch_grid(b);
+        if (rc) return rc;
+        build_work_lists(b, best_cfg.bpi, prog_len);
+        rc = upload_work_lists(b);
+        if (rc) return rc;
+        b->blocks_per_item = best_cfg.bpi;
     }
     // capture the whole run as a graph, replayed by phylo_batch_run
     HIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -2328,6 +2406,25 @@ int phylo_batch_stats(phylo_batch* b, double* updates_full, double* updates_nni,
     if (last_ms) *last_ms = b->last_ms;
     if (sites_per_item) *sites_per_item = 64 * b->blocks_per_item;
     if (n_variants) *n_variants = b->n_var;
+    return PHYLO_OK;
+}
+
+int phylo_batch_set_tiling(phylo_batch* b, int sites_per_item, int category_waves) {
+    if (!b) return PHYLO_ERR_ARG;
+    if (b->finalized) return fail(b, PHYLO_ERR_STATE, "batch already finalized");
+    if (sites_per_item < 64 || sites_per_item > 4096 || sites_per_item % 64)
+        return fail(b, PHYLO_ERR_ARG, "set_tiling: sites per item must be a multiple of 64 up to 4096");
+    if (category_waves != 1 && category_waves != b->C)
+        return fail(b, PHYLO_ERR_ARG, "set_tiling: category waves must be 1 or the number of categories");
+    b->blocks_per_item = sites_per_item / 64;
+    b->cat_waves = category_waves;
+    return PHYLO_OK;
+}
+
+int phylo_batch_tiling(phylo_batch* b, int* sites_per_item, int* category_waves) {
+    if (!b) return PHYLO_ERR_ARG;
+    if (sites_per_item) *sites_per_item = 64 * b->blocks_per_item;
+    if (category_waves) *category_waves = b->cat_waves;
     return PHYLO_OK;
 }
 

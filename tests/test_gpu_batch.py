@@ -53,3 +53,45 @@ def test_batch_matches_single_contexts_and_oracle(oracle, prec):
         assert [(a, s) for a, s, _ in got] == [(a, s) for a, s, _ in single]
         for (_, _, x), (_, _, y) in zip(got, single):
             assert abs(x - y) <= 1e-9 * abs(y) + 1e-9
+
+
+def _run_batch(wls, prec, tiling=None):
+    with P.Batch(4, prec) as b:
+        for i, wl in enumerate(wls):
+            t = wl.tree
+            b.add(wl.masks, t.left, t.right, t.root, t.blen, wl.Q, wl.pi, wl.cat_rates, wl.cat_weights,
+                  nni_mode=i % 3 - 1)
+        if tiling:
+            b.set_tiling(*tiling)
+        b.finalize(autotune=tiling is None)
+        tot, nni = b.run()
+        return tot, nni, b.tiling()
+
+
+@pytest.mark.parametrize("prec", [P.PREC_FP32, P.PREC_FP64])
+def test_batch_tilings_agree(prec):
+    """The category tiling (one category per wave, or one wave walking all
+    four) does not change a bit; work-item sizes change only how per-item sums
+    are grouped (relative 1e-14); the autotuner reports the tiling it chose."""
+    wls = make_instances(7, 5)
+    for spi in (64, 256, 4096):
+        tot4, nni4, got4 = _run_batch(wls, prec, (spi, 4))
+        tot1, nni1, got1 = _run_batch(wls, prec, (spi, 1))
+        assert got4 == (spi, 4) and got1 == (spi, 1)
+        np.testing.assert_array_equal(tot1, tot4)
+        np.testing.assert_array_equal(nni1, nni4)
+        if spi == 64:
+            ref_tot, ref_nni = tot4, nni4
+        np.testing.assert_allclose(tot4, ref_tot, rtol=1e-14)
+        np.testing.assert_allclose(nni4, ref_nni, rtol=1e-14)
+    tot, nni, got = _run_batch(wls, prec)
+    assert got[0] in (64, 128, 256, 512, 1024) and got[1] in (1, 4)
+    np.testing.assert_allclose(tot, ref_tot, rtol=1e-14)
+    np.testing.assert_allclose(nni, ref_nni, rtol=1e-14)
+
+
+def test_batch_rejects_bad_tilings():
+    with P.Batch(4, P.PREC_FP32) as b:
+        for bad in [(0, 4), (100, 4), (8192, 4), (128, 2), (128, 3)]:
+            with pytest.raises(P.PhyloError):
+                b.set_tiling(*bad)
