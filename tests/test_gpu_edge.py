@@ -196,3 +196,77 @@ def test_invalid_inputs_are_rejected():
         zero[0, 0] = 0
         with pytest.raises(P.PhyloError):
             ctx.set_tips(zero, 4)                  # a tip with no state
+
+
+def balanced(depth, blen):
+    """A perfectly balanced tree of 2^depth tips: every internal node above
+    the cherries has two internal children, so half the results of a walk
+    wait for a sibling subtree (carry B or memory)."""
+    n = 1 << depth
+    nn = 2 * n - 1
+    left = -np.ones(nn, np.int32)
+    right = -np.ones(nn, np.int32)
+    level = list(range(n))
+    nxt = n
+    while len(level) > 1:
+        up = []
+        for a, b in zip(level[0::2], level[1::2]):
+            left[nxt], right[nxt] = a, b
+            up.append(nxt)
+            nxt += 1
+        level = up
+    b = np.full(nn, blen)
+    b[nn - 1] = 0.0
+    return _tree(left, right, nn - 1, b, n)
+
+
+def two_caterpillars(k, blen):
+    """Two caterpillars of k tips under one root: long chains that walk with
+    carry A only, then one wait at the root."""
+    n = 2 * k
+    nn = 2 * n - 1
+    left = -np.ones(nn, np.int32)
+    right = -np.ones(nn, np.int32)
+    v = n
+    tops = []
+    for side in range(2):
+        t0 = side * k
+        left[v], right[v] = t0, t0 + 1
+        prev = v
+        v += 1
+        for j in range(2, k):
+            left[v], right[v] = t0 + j, prev     # the chain on the right
+            prev = v
+            v += 1
+        tops.append(prev)
+    left[v], right[v] = tops
+    b = np.full(nn, blen)
+    b[v] = 0.0
+    return _tree(left, right, v, b, n)
+
+
+@pytest.mark.parametrize("shape", ["balanced", "two_caterpillars"])
+@pytest.mark.parametrize("keep", [False, True])
+def test_tree_shapes_against_oracle(oracle, shape, keep):
+    """Shapes at the extremes of the walk's register use, both storing modes,
+    both precisions; NNI scores of the balanced tree against the oracle."""
+    print(f"shape {shape} keep={keep}", flush=True)
+    t = balanced(6, 0.08) if shape == "balanced" else two_caterpillars(20, 0.06)
+    wl = _workload(t, 517, C=4, seed=21)
+    for prec in (0, 1):
+        ref_tot, ref_site = oracle_problem(oracle, wl).loglik()
+        with gpu_context(wl, precision=prec) as ctx:
+            ctx.set_keep_partials(keep)
+            tot, site = ctx.loglik(per_site=True)
+            rel = np.abs(site - ref_site) / np.maximum(np.abs(ref_site), 1.0)
+            assert rel.max() <= (1e-9 if prec else 1e-5), rel.max()
+            if shape == "balanced" and prec == 1:
+                base, var = ctx.nni_score(1)
+                assert abs(base - ref_tot) <= 1e-9 * abs(ref_tot)
+                mb = np.zeros((t.n_nodes, wl.masks.shape[1]), np.uint32)
+                mb[: t.n_tips] = wl.masks
+                for v, side, lnl in var[::7]:
+                    l2, r2 = oracle.nni_apply(t.left, t.right, t.root, v, side)
+                    want, _ = oracle.Problem(l2, r2, t.root, t.blen, mb, 4, wl.Q, wl.pi, pi=wl.pi,
+                                             cat_rates=wl.cat_rates, cat_w=wl.cat_weights).loglik()
+                    assert abs(lnl - want) <= 1e-9 * abs(want), (v, side, lnl, want)
