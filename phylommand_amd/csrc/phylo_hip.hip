@@ -168,6 +168,31 @@ __device__ inline void block_sum_store(double v, double* dst) {
     }
 }
 
+// Bit j of a tip's state set as 1.0 or 0.0 (the high word of 1.0 is 0x3FF00000).
+// (A signed bit-field extract gives 0 or all ones, masked to the exponent
+// bits of 1.0: two instructions where the compiler's select takes three.)
+template <int J>
+__device__ inline double tip_bit(unsigned code) {
+    int t;
+    asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(t) : "v"(code), "i"(J));
+    return __hiloint2double(t & 0x3FF00000, 0);
+}
+
+// frexp exponent of the largest of four values (0 if none is positive): as
+// signed integers the high words of positive doubles order like the values,
+// and those of negative ones (P(t) from the eigen system can leave tiny
+// negative partials) are negative, so they never win.
+__device__ inline int max_exponent(const double (&L)[4]) {
+    const int h = max(max(__double2hiint(L[0]), __double2hiint(L[1])), max(__double2hiint(L[2]), __double2hiint(L[3])));
+    int e = (h >> 20) - 1022;
+    if (h < 0x00100000) {   // no positive normal value: zero, subnormal or negative maximum
+        const double mx = fmax(fmax(L[0], L[1]), fmax(L[2], L[3]));
+        e = 0;
+        if (mx > 0.0) (void)frexp(mx, &e);
+    }
+    return e;
+}
+
 // The pruning walker for S <= 4 (DNA, binary; padded states carry zero rows/cols of P).
 //
 // A workgroup owns 64 consecutive sites; wave c of the workgroup computes rate
@@ -214,8 +239,10 @@ __device__ inline void walk4(const Op* __restrict__ ops, int k0, int k1, const u
             if (op.l_kind == SRC_TIP) {
                 const unsigned code = TL ? (tl[op.l_idx * 32 + (lane >> 1)] >> ((lane & 1) * 4)) & 15u
                                          : tips[(size_t)op.l_idx * m_pad + s];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) x[j] = (code >> j) & 1u ? 1.0 : 0.0;
+                x[0] = tip_bit<0>(code);
+                x[1] = tip_bit<1>(code);
+                x[2] = tip_bit<2>(code);
+                x[3] = tip_bit<3>(code);
             } else if (op.l_kind == SRC_CLV) {
                 load4(plane + op.l_idx * slot_stride, x);
                 sc += scl[((size_t)op.l_idx * C + cat) * m_pad + s];
@@ -243,8 +270,10 @@ __device__ inline void walk4(const Op* __restrict__ ops, int k0, int k1, const u
             if (op.r_kind == SRC_TIP) {
                 const unsigned code = TL ? (tl[op.r_idx * 32 + (lane >> 1)] >> ((lane & 1) * 4)) & 15u
                                          : tips[(size_t)op.r_idx * m_pad + s];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) x[j] = (code >> j) & 1u ? 1.0 : 0.0;
+                x[0] = tip_bit<0>(code);
+                x[1] = tip_bit<1>(code);
+                x[2] = tip_bit<2>(code);
+                x[3] = tip_bit<3>(code);
             } else if (op.r_kind == SRC_CLV) {
                 load4(plane + op.r_idx * slot_stride, x);
                 sc += scl[((size_t)op.r_idx * C + cat) * m_pad + s];
@@ -270,9 +299,7 @@ __device__ inline void walk4(const Op* __restrict__ ops, int k0, int k1, const u
             for (int i = 0; i < 4; ++i) L[i] *= R[i];
         }
         // this category's power-of-two scaler: the partial's maximum to [0.5, 1)
-        const double mx = fmax(fmax(L[0], L[1]), fmax(L[2], L[3]));
-        int e = 0;
-        if (mx > 0.0) (void)frexp(mx, &e);
+        const int e = max_exponent(L);
         sc += e;
 #pragma unroll
         for (int i = 0; i < 4; ++i) L[i] = ldexp(L[i], -e);
@@ -286,14 +313,15 @@ __device__ inline void walk4(const Op* __restrict__ ops, int k0, int k1, const u
             store4(const_cast<T*>(plane) + op.out_slot * slot_stride, r);
             scl[((size_t)op.out_slot * C + cat) * m_pad + s] = sc;
         }
+        // carry A takes every result (after an op to carry B nothing reads
+        // the old A), so no select between the carries
+#pragma unroll
+        for (int i = 0; i < 4; ++i) cA[i] = L[i];
+        sA = sc;
         if (op.flags & OP_TO_B) {
             xb[cat][0][lane] = make_double2(L[0], L[1]);
             xb[cat][1][lane] = make_double2(L[2], L[3]);
             sB = sc;
-        } else {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) cA[i] = L[i];
-            sA = sc;
         }
         if (op.flags & OP_ROOT) {
             double lc = 0.0;
@@ -1496,14 +1524,16 @@ bool append_nni_program(const TV* c, const std::vector<int>& par, int v, int sid
             cur = up;
         }
     } else {
-        // at the root: v is carried in A, its sibling in B, then the root
-        o2.flags = OP_TO_B;
+        // at the root: v waits in B (o2 does not read it), its sibling comes
+        // in A, then the root
+        ops[ops.size() - 1].flags |= OP_TO_B;
+        o2.flags = 0;
         ops.push_back(o2);
         Op o3{};
         o3.l_node = c->left[r];
         o3.r_node = c->right[r];
-        o3.l_kind = c->left[r] == v ? SRC_CARRY_A : SRC_CARRY_B;
-        o3.r_kind = c->right[r] == v ? SRC_CARRY_A : SRC_CARRY_B;
+        o3.l_kind = c->left[r] == v ? SRC_CARRY_B : SRC_CARRY_A;
+        o3.r_kind = c->right[r] == v ? SRC_CARRY_B : SRC_CARRY_A;
         o3.l_idx = o3.r_idx = 0;
         o3.out_slot = -1;
         o3.flags = OP_ROOT;
