@@ -290,7 +290,8 @@ def bench_batch(args, P, synth, n_inst=512):
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic", "config": {"workload": f"{n_inst} GTR+G4 alignments + full NNI neighbourhoods",
                                             "updates_full": u1, "updates_nni": u2, "variants": nv,
-                                            "sites_per_item": spi, "category_waves": b.tiling()[1],
+                                            "sites_per_item": spi, "category_waves": b.launch_shape()[1],
+                                            "workgroups_per_cu": b.launch_shape()[2],
                                             "device_ms": ms,
                                             "finite": bool(np.all(np.isfinite(tot)) and np.all(np.isfinite(nni)))}}
 

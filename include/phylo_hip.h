@@ -188,7 +188,8 @@ int phylo_batch_set_tiling(phylo_batch *b, int sites_per_item, int category_wave
 /* Uploads everything, builds the work lists and captures the graph.  With
  * autotune != 0 it first times 64, 128, 256 and 512 sites per work item with
  * one rate category per wave, then one wave walking every category at the
- * best size and twice it, and keeps the fastest (phylo_batch_tiling). */
+ * best size and twice it, then up to three fewer resident workgroups per CU,
+ * and keeps the fastest (phylo_batch_tiling). */
 int phylo_batch_finalize(phylo_batch *b, int autotune);
 
 /* Replays the graph.  totals: one lnL per problem; nni_lnl (nullable): every
@@ -203,10 +204,10 @@ int phylo_batch_nni_layout(phylo_batch *b, int32_t *offsets, int32_t *node, int3
 int phylo_batch_stats(phylo_batch *b, double *updates_full, double *updates_nni, double *last_ms,
                       int *sites_per_item, int *n_variants);
 
-/* The tiling the autotuner chose: sites per work item and rate-category
- * waves per workgroup (n_cat: one category per wave; 1: one wave walks every
- * category in turn). */
-int phylo_batch_tiling(phylo_batch *b, int *sites_per_item, int *category_waves);
+/* The tiling the autotuner chose: sites per work item, rate-category waves
+ * per workgroup (n_cat: one category per wave; 1: one wave walks every
+ * category in turn) and resident workgroups per CU of the persistent grid. */
+int phylo_batch_tiling(phylo_batch *b, int *sites_per_item, int *category_waves, int *workgroups_per_cu);
 
 void phylo_batch_destroy(phylo_batch *b);
 

@@ -59,7 +59,8 @@ SIGNATURES = [
     ("phylo_batch_stats", _c_int, [_c_void_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.POINTER(ctypes.c_int),
                                    ctypes.POINTER(ctypes.c_int)]),
     ("phylo_batch_set_tiling", _c_int, [_c_void_p, _c_int, _c_int]),
-    ("phylo_batch_tiling", _c_int, [_c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    ("phylo_batch_tiling", _c_int, [_c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                    ctypes.POINTER(ctypes.c_int)]),
     ("phylo_batch_destroy", None, [_c_void_p]),
     ("phylo_last_kernel_ms", _c_int, [_c_void_p, _c_dbl_p]),
     ("phylo_simulate", _c_int, [_c_void_p, _c_int, _c_dbl_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint8)]),
@@ -401,9 +402,13 @@ class Batch:
 
     def tiling(self):
         """(sites per work item, category waves per workgroup) chosen by finalize."""
-        spi, cw = ctypes.c_int(0), ctypes.c_int(0)
-        self._check(self.lib.phylo_batch_tiling(self.h, ctypes.byref(spi), ctypes.byref(cw)))
-        return spi.value, cw.value
+        return self.launch_shape()[:2]
+
+    def launch_shape(self):
+        """(sites per work item, category waves per workgroup, resident workgroups per CU)."""
+        spi, cw, wpc = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self.lib.phylo_batch_tiling(self.h, ctypes.byref(spi), ctypes.byref(cw), ctypes.byref(wpc)))
+        return spi.value, cw.value, wpc.value
 
     def close(self):
         if getattr(self, "h", None):

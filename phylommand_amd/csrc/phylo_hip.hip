@@ -1972,6 +1972,8 @@ struct phylo_batch {
     std::vector<Inst> inst;
     bool finalized = false;
     int cat_waves = 4;                // category tiling: C waves per workgroup, or 1
+    int per_cu_cap = 0;               // resident workgroups per CU of the persistent grid (0: all that fit)
+    int max_per_cu = 1;               // all that fit, for the current tiling
     int blocks_per_item = 2, grid = 0, n_items1 = 0, n_items2 = 0, n_seg1 = 0, n_seg2 = 0, n_var = 0;
     double updates1 = 0, updates2 = 0, last_ms = 0;
     std::vector<int> var_offsets;     // per instance into the flat variant list
@@ -2037,7 +2039,8 @@ int batch_grid(phylo_batch* b) {
 #undef PHYLO_BATCH_KFN
     int per_cu = 1;
     HIPCHK(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 64 * b->cat_waves, 0));
-    b->grid = std::max(1, per_cu) * prop.multiProcessorCount;
+    b->max_per_cu = std::max(1, per_cu);
+    b->grid = std::min(b->max_per_cu, b->per_cu_cap > 0 ? b->per_cu_cap : b->max_per_cu) * prop.multiProcessorCount;
     return PHYLO_OK;
 }
 
@@ -2326,17 +2329,18 @@ int phylo_batch_finalize(phylo_batch* b, int autotune) {
     // autotune the work-item size (64-site blocks per item) with the
     // categories on separate waves, then the one-wave category tiling around
     // the best size
-    struct Cfg { int cw, bpi; };
+    struct Cfg { int cw, bpi, cap; };
     std::vector<Cfg> cands;
     if (autotune) {
-        for (int bpi : {1, 2, 4, 8}) cands.push_back({b->C, bpi});
+        for (int bpi : {1, 2, 4, 8}) cands.push_back({b->C, bpi, 0});
     } else {
-        cands.push_back({b->cat_waves, b->blocks_per_item});
+        cands.push_back({b->cat_waves, b->blocks_per_item, b->per_cu_cap});
     }
     float best = 1e30f;
     Cfg best_cfg = cands.front();
     auto trial = [&](const Cfg& cfg) -> int {
         b->cat_waves = cfg.cw;
+        b->per_cu_cap = cfg.cap;
         int rc = batch_grid(b);
         if (rc) return rc;
         build_work_lists(b, cfg.bpi, prog_len);
@@ -2362,12 +2366,25 @@ int phylo_batch_finalize(phylo_batch* b, int autotune) {
     if (autotune && b->C > 1) {
         const int bpi0 = best_cfg.bpi;
         for (int bpi : {bpi0, 2 * bpi0}) {
-            int rc = trial({1, bpi});
+            int rc = trial({1, bpi, 0});
             if (rc) return rc;
         }
     }
-    if (b->cat_waves != best_cfg.cw || b->blocks_per_item != best_cfg.bpi) {
+    if (autotune) {
+        // fewer resident workgroups per CU: the NNI programs mostly read
+        // stored partials, and more waves in flight can thrash the caches
         b->cat_waves = best_cfg.cw;
+        int rc = batch_grid(b);
+        if (rc) return rc;
+        const Cfg base = best_cfg;
+        for (int cap = b->max_per_cu - 1; cap >= std::max(1, b->max_per_cu - 3); --cap) {
+            rc = trial({base.cw, base.bpi, cap});
+            if (rc) return rc;
+        }
+    }
+    if (b->cat_waves != best_cfg.cw || b->blocks_per_item != best_cfg.bpi || b->per_cu_cap != best_cfg.cap) {
+        b->cat_waves = best_cfg.cw;
+        b->per_cu_cap = best_cfg.cap;
         int rc = batch_grid(b);
         if (rc) return rc;
         build_work_lists(b, best_cfg.bpi, prog_len);
@@ -2451,10 +2468,12 @@ int phylo_batch_set_tiling(phylo_batch* b, int sites_per_item, int category_wave
     return PHYLO_OK;
 }
 
-int phylo_batch_tiling(phylo_batch* b, int* sites_per_item, int* category_waves) {
+int phylo_batch_tiling(phylo_batch* b, int* sites_per_item, int* category_waves, int* workgroups_per_cu) {
     if (!b) return PHYLO_ERR_ARG;
     if (sites_per_item) *sites_per_item = 64 * b->blocks_per_item;
     if (category_waves) *category_waves = b->cat_waves;
+    if (workgroups_per_cu)
+        *workgroups_per_cu = b->per_cu_cap > 0 ? std::min(b->per_cu_cap, b->max_per_cu) : b->max_per_cu;
     return PHYLO_OK;
 }
 
