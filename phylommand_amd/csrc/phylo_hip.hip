@@ -105,8 +105,7 @@ __device__ inline uint32_t mix32(uint32_t x) {
 // that computes it.  (Zero stays zero; values below the float normal range
 // end as the nearest float of the truncated value.)
 // The dithered cut and the power-of-two scaling commute (the cut reads only
-// mantissa bits), so a walker may cut before the scaler exchange, to overlap
-// it, or after it, on the few results it stores.
+// mantissa bits), so a walker may cut before or after scaling a partial.
 __device__ inline double round_cut(double x, uint32_t bits, float) {
     // 64-bit add of the 29 dither bits, written on 32-bit halves
     const uint32_t lo = (uint32_t)__double2loint(x);
