@@ -17,10 +17,28 @@ simpleML::~simpleML() {
 void simpleML::init() {
     tipdata_.assign(nodes_.size(), {});
     site_lnl_.clear();
+    tip_of_taxon_.clear();
+}
+
+// find_taxon_tip (a scan over the tips, tree.cpp:1117-1129 in the reference)
+// through a hash map, so filling m characters of n taxa costs O(n m), not
+// O(n^2 m).  A hit is checked against the tree, a miss rebuilds the map once,
+// so the map can never answer differently from the scan.
+int simpleML::tip_of(const std::string& taxon) {
+    auto valid = [&](int v) {
+        return v >= 0 && (size_t)v < nodes_.size() && is_tip(v) && nodes_[v].has_label && nodes_[v].label == taxon;
+    };
+    auto it = tip_of_taxon_.find(taxon);
+    if (it != tip_of_taxon_.end() && valid(it->second)) return it->second;
+    tip_of_taxon_.clear();
+    for (int v : tips())
+        if (nodes_[v].has_label) tip_of_taxon_.emplace(nodes_[v].label, v);   // first tip of a label wins
+    it = tip_of_taxon_.find(taxon);
+    return it != tip_of_taxon_.end() ? it->second : -1;
 }
 
 void simpleML::set_char(const int char_no, const std::string& taxon, const state_set& state, sub_model& model) {
-    const int leaf = find_taxon_tip(taxon);
+    const int leaf = tip_of(taxon);
     if (leaf < 0 || char_no < 0) return;           // taxa absent from the tree are ignored (simpleML.h:45-46)
     if (tipdata_.size() != nodes_.size()) tipdata_.resize(nodes_.size());
     auto& row = tipdata_[leaf];

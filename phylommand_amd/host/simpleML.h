@@ -17,6 +17,7 @@
 
 #include <cstdint>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "io.h"
@@ -72,7 +73,9 @@ class simpleML : public tree {
     bool prepare(sub_model& model, rate_model& rate_mod, std::vector<int>& lib_of_node);
     void effective_lengths(int v, unsigned int clade_no, double dist, rate_model& rate_mod,
                            std::vector<double>& eff) const;
+    int tip_of(const std::string& taxon);
     std::vector<std::vector<state_set>> tipdata_;   // per node, per character
+    std::unordered_map<std::string, int> tip_of_taxon_;   // find_taxon_tip, cached
     std::vector<double> site_lnl_;
     std::vector<double> cat_rates_{1.0}, cat_w_{1.0};
     std::vector<int> lib_of_node_;
