@@ -192,6 +192,62 @@ __device__ inline int max_exponent(const double (&L)[4]) {
     return e;
 }
 
+// One child's operand for a walker op, by source kind (compile-time).
+template <int K, typename T, int C, bool TL>
+__device__ inline void child4(int idx, const uint8_t* __restrict__ tips, const uint8_t* tl, const T* plane,
+                              size_t slot_stride, const int* scl, int m_pad, int s, int lane, int cat,
+                              const double (&cA)[4], int sA, double2 (&xb)[C][2][64], int sB, double (&x)[4], int& sc) {
+    if (K == SRC_TIP) {
+        const unsigned code = TL ? (tl[idx * 32 + (lane >> 1)] >> ((lane & 1) * 4)) & 15u : tips[(size_t)idx * m_pad + s];
+        x[0] = tip_bit<0>(code);
+        x[1] = tip_bit<1>(code);
+        x[2] = tip_bit<2>(code);
+        x[3] = tip_bit<3>(code);
+    } else if (K == SRC_CLV) {
+        load4(plane + idx * slot_stride, x);
+        sc += scl[((size_t)idx * C + cat) * m_pad + s];
+    } else if (K == SRC_CARRY_A) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) x[j] = cA[j];
+        sc += sA;
+    } else {
+        const double2 a = xb[cat][0][lane], b = xb[cat][1][lane];
+        x[0] = a.x; x[1] = a.y; x[2] = b.x; x[3] = b.y;
+        sc += sB;
+    }
+}
+
+// L = (P_l x_l) * (P_r x_r) for one op whose child kinds are known at compile
+// time.  The walker switches once per op over the (left, right) kinds into
+// one straight-line body each; testing the kinds one by one costs the
+// compiler's flag-and-branch glue on every op (about 10 % of the C2 kernel).
+template <int LK, int RK, typename T, int C, bool TL>
+__device__ inline void children4(const Op& op, const uint8_t* __restrict__ tips, const uint8_t* tl, const T* plane,
+                                 size_t slot_stride, const int* scl, int m_pad, int s, int lane, int cat,
+                                 const double (&cA)[4], int sA, double2 (&xb)[C][2][64], int sB,
+                                 const double (&Pl)[16], const double (&Pr)[16], double (&L)[4], int& sc) {
+    double x[4];
+    child4<LK, T, C, TL>(op.l_idx, tips, tl, plane, slot_stride, scl, m_pad, s, lane, cat, cA, sA, xb, sB, x, sc);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        double a = 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) a = fma(Pl[i * 4 + j], x[j], a);
+        L[i] = a;
+    }
+    if (RK != SRC_NONE) {
+        child4<RK, T, C, TL>(op.r_idx, tips, tl, plane, slot_stride, scl, m_pad, s, lane, cat, cA, sA, xb, sB, x,
+                             sc);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            double a = 0.0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) a = fma(Pr[i * 4 + j], x[j], a);
+            L[i] *= a;
+        }
+    }
+}
+
 // The pruning walker for S <= 4 (DNA, binary; padded states carry zero rows/cols of P).
 //
 // A workgroup owns 64 consecutive sites; wave c of the workgroup computes rate
@@ -230,72 +286,20 @@ __device__ inline void walk4(const Op* __restrict__ ops, int k0, int k1, const u
 #pragma unroll
             for (int i = 0; i < 16; ++i) { Pl[i] = pl[i]; Pr[i] = pr[i]; }
         }
-        // child inputs
-        double L[4], R[4];
+        // child inputs: one straight-line body per (left kind, right kind)
+        double L[4];
         int sc = 0;
-        {
-            double x[4];
-            if (op.l_kind == SRC_TIP) {
-                const unsigned code = TL ? (tl[op.l_idx * 32 + (lane >> 1)] >> ((lane & 1) * 4)) & 15u
-                                         : tips[(size_t)op.l_idx * m_pad + s];
-                x[0] = tip_bit<0>(code);
-                x[1] = tip_bit<1>(code);
-                x[2] = tip_bit<2>(code);
-                x[3] = tip_bit<3>(code);
-            } else if (op.l_kind == SRC_CLV) {
-                load4(plane + op.l_idx * slot_stride, x);
-                sc += scl[((size_t)op.l_idx * C + cat) * m_pad + s];
-            } else if (op.l_kind == SRC_CARRY_A) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) x[j] = cA[j];
-                sc += sA;
-            } else {
-                {
-                    const double2 a = xb[cat][0][lane], b = xb[cat][1][lane];
-                    x[0] = a.x; x[1] = a.y; x[2] = b.x; x[3] = b.y;
-                }
-                sc += sB;
-            }
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                double a = 0.0;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) a = fma(Pl[i * 4 + j], x[j], a);
-                L[i] = a;
-            }
-        }
-        if (op.r_kind != SRC_NONE) {
-            double x[4];
-            if (op.r_kind == SRC_TIP) {
-                const unsigned code = TL ? (tl[op.r_idx * 32 + (lane >> 1)] >> ((lane & 1) * 4)) & 15u
-                                         : tips[(size_t)op.r_idx * m_pad + s];
-                x[0] = tip_bit<0>(code);
-                x[1] = tip_bit<1>(code);
-                x[2] = tip_bit<2>(code);
-                x[3] = tip_bit<3>(code);
-            } else if (op.r_kind == SRC_CLV) {
-                load4(plane + op.r_idx * slot_stride, x);
-                sc += scl[((size_t)op.r_idx * C + cat) * m_pad + s];
-            } else if (op.r_kind == SRC_CARRY_A) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) x[j] = cA[j];
-                sc += sA;
-            } else {
-                {
-                    const double2 a = xb[cat][0][lane], b = xb[cat][1][lane];
-                    x[0] = a.x; x[1] = a.y; x[2] = b.x; x[3] = b.y;
-                }
-                sc += sB;
-            }
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                double a = 0.0;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) a = fma(Pr[i * 4 + j], x[j], a);
-                R[i] = a;
-            }
-#pragma unroll
-            for (int i = 0; i < 4; ++i) L[i] *= R[i];
+        switch (op.l_kind * 5 + op.r_kind) {
+#define PHYLO_SHAPE(A, B)                                                                                   \
+    case A * 5 + B:                                                                                       \
+        children4<A, B, T, C, TL>(op, tips, tl, plane, slot_stride, scl, m_pad, s, lane, cat, cA, sA, xb, sB, Pl, \
+                                  Pr, L, sc);                                                             \
+        break;
+            PHYLO_SHAPE(0, 0) PHYLO_SHAPE(0, 1) PHYLO_SHAPE(0, 2) PHYLO_SHAPE(0, 3) PHYLO_SHAPE(0, 4)
+            PHYLO_SHAPE(1, 0) PHYLO_SHAPE(1, 1) PHYLO_SHAPE(1, 2) PHYLO_SHAPE(1, 3) PHYLO_SHAPE(1, 4)
+            PHYLO_SHAPE(2, 0) PHYLO_SHAPE(2, 1) PHYLO_SHAPE(2, 2) PHYLO_SHAPE(2, 3) PHYLO_SHAPE(2, 4)
+            default: PHYLO_SHAPE(3, 0) PHYLO_SHAPE(3, 1) PHYLO_SHAPE(3, 2) PHYLO_SHAPE(3, 3) PHYLO_SHAPE(3, 4)
+#undef PHYLO_SHAPE
         }
         // this category's power-of-two scaler: the partial's maximum to [0.5, 1)
         const int e = max_exponent(L);
