@@ -132,24 +132,53 @@ __device__ inline double wave_sum(double v) {
 // partial[l], partial[l + 64], ... with Kahan compensation, then the lanes'
 // (sum, error) pairs meet in a two-sum butterfly.  Every lane returns the
 // same total (two-sum gives the exact error whichever operand comes first).
-__device__ inline double wave_total(const double* partial, int n, int lane) {
+// Adds b + bl to the compensated pair (hi, lo) by Knuth's two-sum.
+__device__ inline void two_sum_add(double& hi, double& lo, double b, double bl) {
+    const double t = hi + b;
+    const double bb = t - hi;
+    const double err = (hi - (t - bb)) + (b - bb);
+    hi = t;
+    lo = (lo + bl) + err;
+}
+
+// Threads of the compensated sum: one workgroup per program.
+constexpr int kSumThreads = 256;
+
+// Compensated fixed-order sum of partial[0, n) by one workgroup of
+// kSumThreads.  Each thread issues its loads eight at a time before adding
+// them (Kahan), so their latencies overlap instead of queueing; the threads'
+// (sum, error) pairs then meet by two-sum, in a butterfly within each wave
+// and in LDS across the waves.  Thread 0 returns the total.
+__device__ inline double block_total(const double* partial, int n) {
     double s = 0.0, comp = 0.0;
-    for (int i = lane; i < n; i += 64) {
-        const double x = partial[i] - comp;
-        const double t = s + x;
-        comp = (t - s) - x;
-        s = t;
+    for (int base = threadIdx.x; base < n; base += 8 * kSumThreads) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = base + u * kSumThreads;
+            v[u] = i < n ? partial[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (base + u * kSumThreads < n) {
+                const double x = v[u] - comp;
+                const double t = s + x;
+                comp = (t - s) - x;
+                s = t;
+            }
+        }
     }
     double hi = s, lo = -comp;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const double b = __shfl_xor(hi, o, 64), bl = __shfl_xor(lo, o, 64);
-        const double t = hi + b;
-        const double bb = t - hi;
-        const double err = (hi - (t - bb)) + (b - bb);
-        hi = t;
-        lo = (lo + bl) + err;
-    }
+    for (int o = 32; o > 0; o >>= 1) two_sum_add(hi, lo, __shfl_xor(hi, o, 64), __shfl_xor(lo, o, 64));
+    __shared__ double red_hi[kSumThreads / 64], red_lo[kSumThreads / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) { red_hi[wid] = hi; red_lo[wid] = lo; }
+    __syncthreads();
+    hi = red_hi[0];
+    lo = red_lo[0];
+#pragma unroll
+    for (int w = 1; w < kSumThreads / 64; ++w) two_sum_add(hi, lo, red_hi[w], red_lo[w]);
     return hi + lo;
 }
 
@@ -810,10 +839,10 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
 }
 
 // Compensated fixed-order sum of the per-workgroup partials of each program
-// (one wave per program, wave_total).
-__global__ __launch_bounds__(64) void final_sum_kernel(const double* __restrict__ partial, int n_part,
-                                                      double* __restrict__ out) {
-    const double t = wave_total(partial + (size_t)blockIdx.x * n_part, n_part, threadIdx.x);
+// (one workgroup per program, block_total).
+__global__ __launch_bounds__(kSumThreads) void final_sum_kernel(const double* __restrict__ partial, int n_part,
+                                                               double* __restrict__ out) {
+    const double t = block_total(partial + (size_t)blockIdx.x * n_part, n_part);
     if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
 
@@ -1122,10 +1151,10 @@ bool build_full_ops(const TV* c, std::vector<Op>& ops) {
 }
 
 // Clears the stores of ops[begin, end) whose result no op of that range reads
-// back from memory: results consumed only through the register carry.  In a
-// postorder walk with one carry only nodes with two internal children keep a
-// store (40 of the 127 internal nodes of a 128-taxon Yule tree), so the
-// partials written per evaluation drop by about two thirds.
+// back from memory: results consumed only through the two carries.  With the
+// second carry only results that wait while a branching subtree is walked keep
+// a store (11 of the 127 internal nodes of a 128-taxon Yule tree), so the
+// partials written per evaluation drop by more than 90%.
 void drop_unread_stores(std::vector<Op>& ops, size_t begin, size_t end) {
     std::vector<char> read;
     for (size_t k = begin; k < end; ++k) {
@@ -1400,7 +1429,7 @@ int run_programs(phylo_ctx* c, const Op* d_ops, const int* d_prog, int n_prog, d
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     // the totals go straight into mapped pinned memory: no copy
-    hipLaunchKernelGGL(final_sum_kernel, dim3(n_prog), dim3(64), 0, c->stream, c->d_partial, nblk, c->h_total_dev);
+    hipLaunchKernelGGL(final_sum_kernel, dim3(n_prog), dim3(kSumThreads), 0, c->stream, c->d_partial, nblk, c->h_total_dev);
     HIPCHK(c, hipGetLastError());
     if (site_lnl_host)
         HIPCHK(c, hipMemcpyAsync(site_lnl_host, c->d_sitelnl, (size_t)n_prog * c->m * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1565,7 +1594,7 @@ int phylo_create(int device, phylo_ctx** out) {
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
-        delete c;
+        phylo_destroy(c);
         return PHYLO_ERR_HIP;
     }
     *out = c;
