@@ -8,8 +8,10 @@ equal-weight categories, 5% of cells gap/'N'.  Partials stored in FP32 with
 power-of-two scaling; arithmetic in FP64.
 
 A step is one full likelihood evaluation (phylo_loglik: the pruning launch
-over every internal node and site, the compensated total, and the 8-byte
-copy of the total back to the host) with all inputs resident on the GPU.
+over every internal node and site, the compensated total written to mapped
+host memory, and the wait for it) with all inputs resident on the GPU.  The
+default warm-up (300 evaluations) lets the kernel time settle: over the first
+100 or so evaluations of a fresh process it falls by about 6 %.
 One update = one (site, internal node, rate category) partial:
 (n-1) * m * C = 127 * 100000 * 4 = 5.08e7 per step.
 
@@ -345,8 +347,10 @@ def bench_learn(args, P):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 500 dna/protein, 50 otherwise)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps first (default 300 dna/protein: the kernel time settles after about "
+                         "100 evaluations, about 20 ms of GPU work; 5 otherwise)")
     ap.add_argument("--sites", type=int, default=None, help="sites (default 100,000 DNA, 20,000 protein)")
     ap.add_argument("--taxa", type=int, default=128)
     ap.add_argument("--fp64", action="store_true", help="store partials in FP64")
@@ -360,6 +364,11 @@ def main():
                          "for the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--copy-time", action="store_true", help="with --copy-probe: time the copy, print GB/s")
     args = ap.parse_args()
+    short = args.workload in ("dna", "protein")
+    if args.steps is None:
+        args.steps = 500 if short else 50
+    if args.warmup is None:
+        args.warmup = 300 if short else 5
     if args.copy_probe and args.copy_time:
         print(json.dumps({"copy_gbs": copy_bandwidth(args.copy_probe)}))
         return
