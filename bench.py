@@ -426,6 +426,21 @@ def main():
     elapsed = time.perf_counter() - t0
     assert np.isfinite(lnl) and lnl == lnl0, (lnl, lnl0)
 
+    # End to end, as an optimiser calls it: new edge lengths before every
+    # evaluation (eigen system and lengths copied host -> device, P rebuilt
+    # on the device), then the evaluation and the total back on the host.
+    blens = (t.blen, t.blen * 1.01)
+    e2e_steps = max(20, min(args.steps, 200))
+    seen = [None, None]
+    t1 = time.perf_counter()
+    for i in range(e2e_steps):
+        ctx.set_edge_lengths(blens[i & 1])
+        v = ctx.loglik()
+        assert seen[i & 1] is None or v == seen[i & 1], (i, v, seen[i & 1])
+        seen[i & 1] = v
+    e2e_ms = (time.perf_counter() - t1) / e2e_steps * 1e3
+    ctx.set_edge_lengths(t.blen)
+
     n, m, C = t.n_tips, args.sites, 4
     updates = (n - 1) * m * C
     value = updates * args.steps / elapsed
@@ -449,6 +464,9 @@ def main():
         "config": {"workload": f"GTR+G4 DNA, {n} taxa Yule, {m} sites, 5% gaps",
                    "n_taxa": n, "n_sites": m, "n_cat": C, "clv_storage": "f64" if args.fp64 else "f32",
                    "arithmetic": "f64", "lnL": lnl},
+        "end_to_end": {"ms_per_step": e2e_ms, "steps": e2e_steps,
+                       "what": "set_edge_lengths with new lengths (host-to-device copy of the eigen system "
+                               "and edge lengths, P rebuilt on the device) + loglik, alternating two length sets"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "prune4_kernel<float,4,true>", "kernel_ms": kernel_s * 1e3,
