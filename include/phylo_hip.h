@@ -101,6 +101,15 @@ int phylo_set_categories(phylo_ctx *ctx, int n_cat, const double *rates, const d
  * partial in simpleML's likelihoods map (simpleML.h:105). */
 int phylo_set_keep_partials(phylo_ctx *ctx, int keep);
 
+/* Whether a whole-tree DNA evaluation whose 64-site blocks do not divide
+ * evenly among the GPU's CUs walks the blocks past the last full round (the
+ * tail) as subtree pieces beside the other blocks plus a short second launch
+ * for the nodes above the cut (enable != 0, the default), or as whole-tree
+ * workgroups.  Values differ only by the FP32 rounding of the extra stored
+ * partials of the tail sites (FP64 storage: bit-identical).  No counterpart
+ * in the reference (a launch-shape choice); kept switchable for tests. */
+int phylo_set_tail_split(phylo_ctx *ctx, int enable);
+
 /* One full postorder evaluation.  site_lnl (nullable): n_sites values.
  * total: weighted sum of the site log-likelihoods (compensated FP64 sum).
  * Replaces: simpleML::calculate_log_likelihood(sub_model&[, rate_model&])

@@ -35,6 +35,7 @@ SIGNATURES = [
     ("phylo_last_error", ctypes.c_char_p, [_c_void_p]),
     ("phylo_set_precision", _c_int, [_c_void_p, _c_int]),
     ("phylo_set_keep_partials", _c_int, [_c_void_p, _c_int]),
+    ("phylo_set_tail_split", _c_int, [_c_void_p, _c_int]),
     ("phylo_set_tips", _c_int, [_c_void_p, _c_int, _c_int, _c_int, ctypes.POINTER(ctypes.c_uint32)]),
     ("phylo_set_site_weights", _c_int, [_c_void_p, _c_dbl_p]),
     ("phylo_set_topology", _c_int, [_c_void_p, _c_int, _c_int, _c_int_p, _c_int_p]),
@@ -224,6 +225,12 @@ class Context:
         the walk reads back; node_partials / nni_score evaluate again when
         they need the rest)."""
         self._check(self.lib.phylo_set_keep_partials(self.h, int(bool(keep))))
+
+    def set_tail_split(self, enable: bool):
+        """Walk the tail of a whole-tree DNA evaluation (site blocks past the
+        last full round over the CUs) as subtree pieces plus a short second
+        launch (default), or as whole-tree workgroups."""
+        self._check(self.lib.phylo_set_tail_split(self.h, int(bool(enable))))
 
     def set_tips(self, masks: np.ndarray, n_states: int):
         masks = np.ascontiguousarray(masks, dtype=np.uint32)

@@ -389,7 +389,13 @@ __device__ inline double combine_categories(double lc, int sc, int cat, int lane
 // Taxa whose tip codes a DNA workgroup stages in LDS (4 bits per site).
 constexpr int kTipLdsTaxa = 256;
 
-// grid = (m_pad / 64, n_programs); program y runs ops[prog_begin[y] .. prog_begin[y+1]).
+// Plain launch (prog_main < 0): grid = (m_pad / 64, n_programs); workgroup
+// (x, y) runs ops[prog_begin[y] .. prog_begin[y+1]) over site block x.
+// Split launch (prog_main >= 0, grid.y = 1; build_split_programs): workgroups
+// x < n_main run program prog_main over site block blk0 + x; the others run
+// piece program prog_piece + (x - n_main) / n_tail over site block
+// tail0 + (x - n_main) % n_tail, which has no root: they store their results
+// and leave.
 // TL (n_taxa <= kTipLdsTaxa): the workgroup first copies its 64 sites of
 // every tip into LDS, so no op of the walk waits on a global tip load.
 template <typename T, int C, bool TL>
@@ -397,17 +403,30 @@ __global__ __launch_bounds__(64 * C) void prune4_kernel(
     const Op* __restrict__ ops, const int* __restrict__ prog_begin, const uint8_t* __restrict__ tips,
     T* clv, int* scl, const double* __restrict__ P, const double* __restrict__ roots,
     const double* __restrict__ site_w, double* __restrict__ site_lnl, double* __restrict__ partial,
-    int m, int m_pad, int n_taxa) {
+    int m, int m_pad, int n_taxa, int n_main, int prog_main, int blk0, int prog_piece, int tail0, int n_tail) {
     __shared__ int xsc[C][64];
     __shared__ double xroot[C][64];
     __shared__ double2 xb[C][2][64];
     __shared__ uint8_t tl[TL ? kTipLdsTaxa * 32 : 1];
     const int lane = threadIdx.x & 63;
     const int cat = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int s = blockIdx.x * 64 + lane;
-    const int prog = blockIdx.y;
+    const int y = blockIdx.y;
+    int bx = blockIdx.x, prog = y;
+    bool has_root = true;
+    if (prog_main >= 0) {
+        if ((int)blockIdx.x < n_main) {
+            bx = blk0 + (int)blockIdx.x;
+            prog = prog_main;
+        } else {
+            const int r = (int)blockIdx.x - n_main;
+            bx = tail0 + r % n_tail;
+            prog = prog_piece + r / n_tail;
+            has_root = false;
+        }
+    }
+    const int s = bx * 64 + lane;
     if (TL) {
-        const uint8_t* blk = tips + (size_t)blockIdx.x * 64;
+        const uint8_t* blk = tips + (size_t)bx * 64;
         for (int i = threadIdx.x; i < n_taxa * 32; i += 64 * C) {
             const uint32_t pr = *reinterpret_cast<const uint16_t*>(blk + (size_t)(i >> 5) * m_pad + 2 * (i & 31));
             tl[i] = (uint8_t)((pr & 15u) | ((pr >> 4) & 0xF0u));
@@ -418,11 +437,13 @@ __global__ __launch_bounds__(64 * C) void prune4_kernel(
     int sc;
     walk4<T, C, TL>(ops, prog_begin[prog], prog_begin[prog + 1], tips, tl, clv, scl, P, roots, m_pad, s, lane, cat, xb,
                     lc, sc);
+    if (!has_root) return;    // a piece: its stores are its result (uniform per workgroup)
     const double lnl = combine_categories<C>(lc, sc, cat, lane, xroot, xsc);
     const bool live = s < m && cat == 0;
-    if (live && site_lnl) site_lnl[(size_t)prog * m + s] = lnl;
+    if (live && site_lnl) site_lnl[(size_t)y * m + s] = lnl;
     const double w = live ? (site_w ? site_w[s] : 1.0) : 0.0;
-    block_sum_store(live ? w * lnl : 0.0, partial + (size_t)prog * gridDim.x + blockIdx.x);
+    block_sum_store(live ? w * lnl : 0.0,
+                    partial + (prog_main >= 0 ? (size_t)bx : (size_t)y * gridDim.x + blockIdx.x));
 }
 
 // Batched scoring: many independent problems (and their NNI programs) in one
@@ -958,6 +979,12 @@ struct phylo_ctx {
     std::vector<Op> ops;
     std::vector<int> prog_begin;
     int n_slots = 0;
+    // tail split of whole-tree DNA walks (build_split_programs): split_K chunk
+    // programs from program id split_base[full ? 0 : 1], then one top program
+    int split_K = 0, split_base[2] = {0, 0};
+    bool tail_split = true;                               // phylo_set_tail_split
+    int n_cu = 0;                                         // compute units of the device
+    bool run_full = false;                                // the whole-tree launch stores every partial
 
     // device buffers
     void* d_tips = nullptr; size_t tips_bytes = 0;     // uint8 codes (S <= 4) or uint32 masks
@@ -1174,6 +1201,97 @@ void drop_unread_stores(std::vector<Op>& ops, size_t begin, size_t end) {
 // Two programs over the same postorder: 0 stores every internal node's
 // partial (needed by node_partials and the NNI partial recompute), 1 stores
 // only what the walk itself reads back.  Both compute identical values.
+// The tail split of a whole-tree DNA walk.  When the 64-site blocks do not
+// divide evenly among the CUs, the blocks past the last full round (the
+// tail) put one more workgroup on some CUs, and those CUs set the time of the
+// whole launch (128-taxon GTR+G4 tree: 1,536 blocks on 256 CUs 0.128 ms,
+// 1,564 blocks 0.162 ms).  The tail blocks instead walk the tree in pieces:
+// the tree is cut into subtrees of at most L internal nodes, packed in
+// postorder into chunks of at most L ops whose walks run as extra workgroups
+// beside the main blocks and store their results; a second launch then walks
+// the few nodes above the cut (the top program) for the tail blocks only.
+// Chunk and top programs use carry A alone: a result is stored unless the
+// next op reads it (every result in the storing variant).  Appends, for the
+// storing and the lean evaluation, split_K chunk programs and one top program
+// to c->ops / c->prog_begin; split_K stays 0 when no cut keeps the top
+// program short (a caterpillar tree, say) or the tree is small.
+void build_split_programs(phylo_ctx* c) {
+    c->split_K = 0;
+    const int nt = c->n_taxa, nn = c->n_nodes, root = c->root;
+    const std::vector<int> post = internal_postorder(c, c->left, c->right);
+    const int ni = (int)post.size();
+    if (ni < 16) return;
+    std::vector<int> size(nn, 0), par(nn, -1);
+    for (int i = 0; i < ni; ++i) {
+        const int v = post[i], l = c->left[v], r = c->right[v];
+        size[v] = 1 + (l >= nt ? size[l] : 0) + (r >= nt ? size[r] : 0);
+        if (l >= 0) par[l] = v;
+        if (r >= 0) par[r] = v;
+    }
+    // pieces: the largest subtrees of at most L internal nodes, in postorder;
+    // a subtree's internal nodes are the size[v] entries of post ending at v
+    const auto cut = [&](int L, std::vector<std::vector<int>>& chunks, std::vector<int>& top) {
+        chunks.clear();
+        top.clear();
+        std::vector<int> cur;
+        int cur_n = 0;
+        for (int i = 0; i < ni; ++i) {
+            const int v = post[i];
+            if (size[v] > L) { top.push_back(v); continue; }
+            if (v != root && size[par[v]] <= L) continue;     // inside a larger piece
+            if (cur_n > 0 && cur_n + size[v] > L) { chunks.push_back(cur); cur.clear(); cur_n = 0; }
+            for (int k = i - size[v] + 1; k <= i; ++k) cur.push_back(post[k]);
+            cur_n += size[v];
+        }
+        if (cur_n > 0) chunks.push_back(cur);
+    };
+    int best_L = -1;
+    double best_cost = 1e30;
+    std::vector<std::vector<int>> chunks;
+    std::vector<int> top;
+    for (int d : {16, 12, 10, 8, 6, 5, 4, 3}) {
+        const int L = std::max(1, ni / d);
+        if (size[root] <= L) continue;
+        cut(L, chunks, top);
+        size_t big = 0;
+        for (const auto& ch : chunks) big = std::max(big, ch.size());
+        if (chunks.size() < 2 || top.size() > 16) continue;
+        // the top program runs alone (about one op latency each); the largest
+        // chunk adds to the CUs that take a piece
+        const double cost = (double)top.size() + (double)big / 5.0;
+        if (cost < best_cost) { best_cost = cost; best_L = L; }
+    }
+    if (best_L < 0) return;
+    cut(best_L, chunks, top);
+    const auto emit = [&](const std::vector<int>& nodes, bool store_all) {
+        for (size_t i = 0; i < nodes.size(); ++i) {
+            const int v = nodes[i], l = c->left[v], r = c->right[v];
+            const auto src = [&](int ch, int& kind, int& idx) {
+                if (ch < 0) { kind = SRC_NONE; idx = 0; }
+                else if (ch < nt) { kind = SRC_TIP; idx = ch; }
+                else if (i > 0 && nodes[i - 1] == ch) { kind = SRC_CARRY_A; idx = 0; }
+                else { kind = SRC_CLV; idx = ch - nt; }
+            };
+            Op op{};
+            op.l_node = l;
+            op.r_node = r < 0 ? 0 : r;
+            src(l, op.l_kind, op.l_idx);
+            src(r, op.r_kind, op.r_idx);
+            const bool next_reads = i + 1 < nodes.size() && par[v] == nodes[i + 1];
+            op.out_slot = (store_all || !next_reads) ? v - nt : -1;
+            op.flags = v == root ? OP_ROOT : 0;
+            c->ops.push_back(op);
+        }
+        c->prog_begin.push_back((int)c->ops.size());
+    };
+    c->split_K = (int)chunks.size();
+    for (int mode = 0; mode < 2; ++mode) {     // 0: store every partial, 1: lean
+        c->split_base[mode] = (int)c->prog_begin.size() - 1;
+        for (const auto& ch : chunks) emit(ch, mode == 0);
+        emit(top, mode == 0);
+    }
+}
+
 int compile_full_program(phylo_ctx* c) {
     c->ops.clear();
     if (!build_full_ops(c, c->ops)) return fail(c, PHYLO_ERR_ARG, "topology: internal nodes unreachable from the root");
@@ -1182,6 +1300,7 @@ int compile_full_program(phylo_ctx* c) {
     drop_unread_stores(c->ops, n, 2 * n);
     c->prog_begin = {0, (int)n, (int)(2 * n)};
     c->n_slots = c->n_nodes - c->n_taxa;
+    build_split_programs(c);
     return PHYLO_OK;
 }
 
@@ -1332,13 +1451,29 @@ void launch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
     const double* sw = c->have_site_w ? c->d_sitew : nullptr;
     // staging every tip pays off for whole-tree walks, not for the short
     // path-to-root programs of an NNI neighbourhood
-    if (c->run_whole_tree && c->n_taxa <= kTipLdsTaxa)
-        hipLaunchKernelGGL((prune4_kernel<T, C, true>), grid, dim3(64 * C), 0, c->stream, c->run_ops, c->run_prog, tips,
-                           clv, c->d_scl, c->d_P, c->d_roots, sw, site_lnl, c->d_partial, c->m, c->m_pad, c->n_taxa);
-    else
+    if (c->run_whole_tree && c->n_taxa <= kTipLdsTaxa) {
+        const int nblk = (int)grid.x;
+        const int main_b = c->n_cu > 0 ? nblk / c->n_cu * c->n_cu : 0, tail = nblk - main_b;
+        if (c->tail_split && c->split_K > 0 && grid.y == 1 && main_b > 0 && tail > 0) {
+            // the main blocks and the tail's pieces, then the tail's top program
+            const int base = c->split_base[c->run_full ? 0 : 1], K = c->split_K;
+            hipLaunchKernelGGL((prune4_kernel<T, C, true>), dim3(main_b + K * tail), dim3(64 * C), 0, c->stream,
+                               c->run_ops, c->d_prog, tips, clv, c->d_scl, c->d_P, c->d_roots, sw, site_lnl,
+                               c->d_partial, c->m, c->m_pad, c->n_taxa, main_b, c->run_full ? 0 : 1, 0, base,
+                               main_b, tail);
+            hipLaunchKernelGGL((prune4_kernel<T, C, true>), dim3(tail), dim3(64 * C), 0, c->stream, c->run_ops,
+                               c->d_prog, tips, clv, c->d_scl, c->d_P, c->d_roots, sw, site_lnl, c->d_partial, c->m,
+                               c->m_pad, c->n_taxa, tail, base + K, main_b, 0, 0, 1);
+        } else {
+            hipLaunchKernelGGL((prune4_kernel<T, C, true>), grid, dim3(64 * C), 0, c->stream, c->run_ops, c->run_prog,
+                               tips, clv, c->d_scl, c->d_P, c->d_roots, sw, site_lnl, c->d_partial, c->m, c->m_pad,
+                               c->n_taxa, (int)grid.x, -1, 0, 0, 0, 1);
+        }
+    } else {
         hipLaunchKernelGGL((prune4_kernel<T, C, false>), grid, dim3(64 * C), 0, c->stream, c->run_ops, c->run_prog,
                            tips, clv, c->d_scl, c->d_P, c->d_roots, sw, site_lnl, c->d_partial, c->m, c->m_pad,
-                           c->n_taxa);
+                           c->n_taxa, (int)grid.x, -1, 0, 0, 0, 1);
+    }
 }
 
 template <typename T>
@@ -1592,7 +1727,8 @@ int phylo_create(int device, phylo_ctx** out) {
     phylo_ctx* c = new phylo_ctx();
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
         phylo_destroy(c);
         return PHYLO_ERR_HIP;
     }
@@ -1765,6 +1901,7 @@ int evaluate(phylo_ctx* c, bool full, double* site_lnl, double* total) {
     if (rc) return rc;
     c->clv_valid = false;
     c->run_whole_tree = true;
+    c->run_full = full;
     rc = run_programs(c, c->d_ops, c->d_prog + (full ? 0 : 1), 1, total, site_lnl);
     c->run_whole_tree = false;
     if (rc) return rc;
@@ -1773,6 +1910,13 @@ int evaluate(phylo_ctx* c, bool full, double* site_lnl, double* total) {
     return PHYLO_OK;
 }
 }  // namespace
+
+int phylo_set_tail_split(phylo_ctx* c, int enable) {
+    if (!c) return PHYLO_ERR_ARG;
+    c->tail_split = enable != 0;
+    c->clv_valid = false;      // the tail's stored partials depend on the walk used
+    return PHYLO_OK;
+}
 
 int phylo_set_keep_partials(phylo_ctx* c, int keep) {
     if (!c) return PHYLO_ERR_ARG;

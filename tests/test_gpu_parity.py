@@ -134,3 +134,55 @@ def test_node_partials_follow_changed_inputs(oracle):
             want = clv_o[v] * np.ldexp(1.0, sc_o[v])[:, None, None]
             scale = np.abs(want).max(axis=(1, 2), keepdims=True)
             np.testing.assert_allclose(got / scale, want / scale, atol=1e-10, err_msg=f"node {v}")
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_tail_split_matches_whole_tree_walk(oracle, prec):
+    """20,000 sites are 313 blocks of 64: past a full round over the GPU's CUs
+    the tail blocks walk the tree as subtree pieces plus a short top program.
+    Per-site values against the oracle; lean and storing evaluations agree bit
+    for bit; against the whole-tree walk of the tail (phylo_set_tail_split(0))
+    FP64 storage is bit-identical and FP32 storage differs only by the
+    rounding of the extra stored partials.  Node partials and NNI scores,
+    which read the stored partials, follow the same bars."""
+    print(f"tail split prec={prec}", flush=True)
+    wl = synth.gtr_gamma_workload(n_tips=128, m=20000, C=4, alpha=0.5, gap_frac=0.05, seed=21)
+    ref_tot, ref_site = oracle_problem(oracle, wl).loglik()
+    t = wl.tree
+    nodes = [t.root, t.left[t.root], t.n_tips + 3, t.n_nodes - 2]
+    res = {}
+    for split in (True, False):
+        for keep in (False, True):
+            with gpu_context(wl, precision=prec) as ctx:
+                ctx.set_tail_split(split)
+                ctx.set_keep_partials(keep)
+                tot, site = ctx.loglik(per_site=True)
+                parts = [ctx.node_partials(v, 4, 4) for v in nodes]
+                _, var = ctx.nni_score(1)
+                res[(split, keep)] = (tot, site, parts, var)
+    for (split, keep), (tot, site, _, _) in res.items():
+        rel = np.abs(site - ref_site) / np.abs(ref_site)
+        assert rel.max() <= (1e-9 if prec == 1 else 1e-5), (split, keep, rel.max())
+        assert abs(tot - ref_tot) <= (1e-9 * abs(ref_tot) if prec == 1 else 1e-3), (split, keep, tot - ref_tot)
+    for split in (True, False):
+        a, b = res[(split, False)], res[(split, True)]
+        assert a[0] == b[0]
+        np.testing.assert_array_equal(a[1], b[1])
+        np.testing.assert_array_equal(a[3], b[3])
+    s_, w_ = res[(True, True)], res[(False, True)]
+    if prec == 1:
+        assert s_[0] == w_[0]
+        np.testing.assert_array_equal(s_[1], w_[1])
+        np.testing.assert_array_equal(s_[3], w_[3])
+        for (ca, sa), (cb, sb) in zip(s_[2], w_[2]):
+            np.testing.assert_array_equal(ca, cb)
+            np.testing.assert_array_equal(sa, sb)
+    else:
+        np.testing.assert_allclose(s_[1], w_[1], rtol=1e-6)
+        assert abs(s_[0] - w_[0]) <= 1e-3
+        np.testing.assert_allclose(s_[3], w_[3], rtol=0, atol=1e-3)
+        for (ca, sa), (cb, sb) in zip(s_[2], w_[2]):
+            ga = ca * np.ldexp(1.0, sa)[:, None, None]
+            gb = cb * np.ldexp(1.0, sb)[:, None, None]
+            scale = np.abs(gb).max(axis=(1, 2), keepdims=True)
+            np.testing.assert_allclose(ga / scale, gb / scale, atol=1e-6)
