@@ -470,6 +470,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "prune4_kernel<float,4,true>", "kernel_ms": kernel_s * 1e3,
+                     # kernel_ms spans both launches of an evaluation when the site
+                     # blocks leave a tail over the CUs (main blocks + subtree pieces,
+                     # then the tail's top program), gap included
                      "alg_bytes_per_update": bpu,
                      # the algorithmic bytes assume a CLV store per node: every internal
                      # partial written once and every stored child read back.  The kernel

@@ -699,7 +699,9 @@ __device__ inline void mfma_child(int kind, int idx, int node, int cat, const in
 }
 
 // grid = (m_pad / (16 G), n_programs), block = 64 * C: wave c computes
-// category c of the G 16-site groups starting at group G * blockIdx.x.
+// category c of the G 16-site groups starting at group G * x of site block
+// x (blockIdx.x in a plain launch; the tail-split mapping of prune4_kernel
+// otherwise).
 // Taxa whose tip masks a wide-state workgroup stages in LDS.
 constexpr int kMfmaTipLdsTaxa = 128;
 
@@ -710,12 +712,27 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
     const Op* __restrict__ ops, const int* __restrict__ prog_begin, const uint32_t* __restrict__ tips, T* clv,
     int* scl, const double* __restrict__ Pm, const double* __restrict__ PTm, const double* __restrict__ roots,
     const double* __restrict__ site_w, double* __restrict__ site_lnl, double* __restrict__ partial, int m, int m_pad,
-    int S, uint32_t full, int n_taxa) {
+    int S, uint32_t full, int n_taxa, int n_main, int prog_main, int blk0, int prog_piece, int tail0, int n_tail) {
     __shared__ int xsc[C][16 * G];
     __shared__ double xroot[C][16 * G];
     __shared__ uint32_t tl[TL ? kMfmaTipLdsTaxa * 16 * G : 1];
+    // work mapping as in prune4_kernel (plain or tail-split launch)
+    const int y = blockIdx.y;
+    int bx = blockIdx.x, prog = y;
+    bool has_root = true;
+    if (prog_main >= 0) {
+        if ((int)blockIdx.x < n_main) {
+            bx = blk0 + (int)blockIdx.x;
+            prog = prog_main;
+        } else {
+            const int r = (int)blockIdx.x - n_main;
+            bx = tail0 + r % n_tail;
+            prog = prog_piece + r / n_tail;
+            has_root = false;
+        }
+    }
     if (TL) {
-        const uint32_t* blk = tips + (size_t)blockIdx.x * 16 * G;
+        const uint32_t* blk = tips + (size_t)bx * 16 * G;
         for (int i = threadIdx.x; i < n_taxa * 16 * G; i += 64 * C)
             tl[i] = blk[(size_t)(i / (16 * G)) * m_pad + i % (16 * G)];
         __syncthreads();
@@ -723,11 +740,10 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, j = lane & 15;
     const int cat = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int sg0 = blockIdx.x * G, n_sg = m_pad >> 4;
+    const int sg0 = bx * G, n_sg = m_pad >> 4;
     int s[G];
 #pragma unroll
     for (int q = 0; q < G; ++q) s[q] = (sg0 + q) * 16 + j;
-    const int prog = blockIdx.y;
     const int k0 = prog_begin[prog], k1 = prog_begin[prog + 1];
     const size_t slot_stride = (size_t)C * m_pad * 4 * V;
     // carried results are rounded to T like stored ones (registers are the
@@ -848,14 +864,15 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
             for (int q = 0; q < G; ++q) lnl[q] = log(lc[q]) + sc[q] * kLn2;
         }
     }
+    if (!has_root) return;    // a piece: its stores are its result (uniform per workgroup)
     double sum = 0.0;
 #pragma unroll
     for (int q = 0; q < G; ++q) {
         const bool live = s[q] < m && cat == 0 && g == 0;
-        if (live && site_lnl) site_lnl[(size_t)prog * m + s[q]] = lnl[q];
+        if (live && site_lnl) site_lnl[(size_t)y * m + s[q]] = lnl[q];
         if (live) sum += (site_w ? site_w[s[q]] : 1.0) * lnl[q];
     }
-    block_sum_store(sum, partial + (size_t)prog * gridDim.x + blockIdx.x);
+    block_sum_store(sum, partial + (prog_main >= 0 ? (size_t)bx : (size_t)y * gridDim.x + blockIdx.x));
 }
 
 // Compensated fixed-order sum of the per-workgroup partials of each program
@@ -1249,16 +1266,18 @@ void build_split_programs(phylo_ctx* c) {
     double best_cost = 1e30;
     std::vector<std::vector<int>> chunks;
     std::vector<int> top;
-    for (int d : {16, 12, 10, 8, 6, 5, 4, 3}) {
+    for (int d : {16, 12, 10, 8, 6, 5, 4, 3, 2}) {
         const int L = std::max(1, ni / d);
         if (size[root] <= L) continue;
         cut(L, chunks, top);
         size_t big = 0;
         for (const auto& ch : chunks) big = std::max(big, ch.size());
         if (chunks.size() < 2 || top.size() > 16) continue;
-        // the top program runs alone (about one op latency each); the largest
-        // chunk adds to the CUs that take a piece
-        const double cost = (double)top.size() + (double)big / 5.0;
+        // the top program runs alone, about 0.7 us per op; a chunk shares a
+        // CU with main blocks and costs little more as it grows (C2 tree:
+        // top 2 / chunks <= 56 ops 0.137 ms, top 6 / <= 42 0.139, top 9 /
+        // <= 20 0.141, top 15 / <= 15 0.145)
+        const double cost = (double)top.size() + (double)big / 40.0;
         if (cost < best_cost) { best_cost = cost; best_L = L; }
     }
     if (best_L < 0) return;
@@ -1461,7 +1480,8 @@ void launch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
                                c->run_ops, c->d_prog, tips, clv, c->d_scl, c->d_P, c->d_roots, sw, site_lnl,
                                c->d_partial, c->m, c->m_pad, c->n_taxa, main_b, c->run_full ? 0 : 1, 0, base,
                                main_b, tail);
-            hipLaunchKernelGGL((prune4_kernel<T, C, true>), dim3(tail), dim3(64 * C), 0, c->stream, c->run_ops,
+            // (the top program reads a few tips: no LDS staging)
+            hipLaunchKernelGGL((prune4_kernel<T, C, false>), dim3(tail), dim3(64 * C), 0, c->stream, c->run_ops,
                                c->d_prog, tips, clv, c->d_scl, c->d_P, c->d_roots, sw, site_lnl, c->d_partial, c->m,
                                c->m_pad, c->n_taxa, tail, base + K, main_b, 0, 0, 1);
         } else {
@@ -1498,14 +1518,32 @@ void launch_mfma(phylo_ctx* c, dim3 grid, double* site_lnl) {
     T* clv = reinterpret_cast<T*>(c->d_clv);
     const double* sw = c->have_site_w ? c->d_sitew : nullptr;
     const uint32_t full = c->S == 32 ? 0xFFFFFFFFu : ((1u << c->S) - 1u);
-    if (c->run_whole_tree && c->n_taxa <= kMfmaTipLdsTaxa)
-        hipLaunchKernelGGL((prune_mfma_kernel<T, C, V, kMfmaGroups, true>), grid, dim3(64 * C), 0, c->stream,
-                           c->run_ops, c->run_prog, tips, clv, c->d_scl, c->d_Pm, c->d_PTm, c->d_roots, sw, site_lnl,
-                           c->d_partial, c->m, c->m_pad, c->S, full, c->n_taxa);
-    else
+    if (c->run_whole_tree && c->n_taxa <= kMfmaTipLdsTaxa) {
+        const int nblk = (int)grid.x;
+        const int main_b = c->n_cu > 0 ? nblk / c->n_cu * c->n_cu : 0, tail = nblk - main_b;
+        if (c->tail_split && c->split_K > 0 && grid.y == 1 && main_b > 0 && tail > 0) {
+            // the main blocks and the tail's pieces, then the tail's top program
+            const int base = c->split_base[c->run_full ? 0 : 1], K = c->split_K;
+            hipLaunchKernelGGL((prune_mfma_kernel<T, C, V, kMfmaGroups, true>), dim3(main_b + K * tail), dim3(64 * C),
+                               0, c->stream, c->run_ops, c->d_prog, tips, clv, c->d_scl, c->d_Pm, c->d_PTm,
+                               c->d_roots, sw, site_lnl, c->d_partial, c->m, c->m_pad, c->S, full, c->n_taxa, main_b,
+                               c->run_full ? 0 : 1, 0, base, main_b, tail);
+            // (the top program reads a few tips: no LDS staging)
+            hipLaunchKernelGGL((prune_mfma_kernel<T, C, V, kMfmaGroups, false>), dim3(tail), dim3(64 * C), 0,
+                               c->stream, c->run_ops, c->d_prog, tips, clv, c->d_scl, c->d_Pm, c->d_PTm, c->d_roots,
+                               sw, site_lnl, c->d_partial, c->m, c->m_pad, c->S, full, c->n_taxa, tail, base + K,
+                               main_b, 0, 0, 1);
+        } else {
+            hipLaunchKernelGGL((prune_mfma_kernel<T, C, V, kMfmaGroups, true>), grid, dim3(64 * C), 0, c->stream,
+                               c->run_ops, c->run_prog, tips, clv, c->d_scl, c->d_Pm, c->d_PTm, c->d_roots, sw,
+                               site_lnl, c->d_partial, c->m, c->m_pad, c->S, full, c->n_taxa, (int)grid.x, -1, 0, 0, 0,
+                               1);
+        }
+    } else {
         hipLaunchKernelGGL((prune_mfma_kernel<T, C, V, kMfmaGroups, false>), grid, dim3(64 * C), 0, c->stream,
                            c->run_ops, c->run_prog, tips, clv, c->d_scl, c->d_Pm, c->d_PTm, c->d_roots, sw, site_lnl,
-                           c->d_partial, c->m, c->m_pad, c->S, full, c->n_taxa);
+                           c->d_partial, c->m, c->m_pad, c->S, full, c->n_taxa, (int)grid.x, -1, 0, 0, 0, 1);
+    }
 }
 
 template <typename T, int C>
