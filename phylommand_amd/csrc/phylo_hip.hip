@@ -196,6 +196,30 @@ __device__ inline void block_sum_store(double v, double* dst) {
     }
 }
 
+// The top launch of a tail split (its workgroups are the last blocks to
+// finish an evaluation) folds in the final sum: each workgroup, once its
+// partial is stored, counts itself in; the last one sums every block's
+// partial in block_total's order (so the total equals final_sum_kernel's),
+// writes it to the host-mapped total and returns the counter to zero.  The
+// fences publish the few partials this launch wrote and drop stale copies
+// before the sum reads them.  Workgroups of kSumThreads threads only.
+__device__ inline void fold_total(const double* partial, int n_part, unsigned* counter, double* total) {
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(counter, 1u) == gridDim.x - 1;
+        if (last) __threadfence();
+    }
+    __syncthreads();
+    if (last) {
+        const double t = block_total(partial, n_part);
+        if (threadIdx.x == 0) {
+            *total = t;
+            *counter = 0u;
+        }
+    }
+}
+
 // Bit j of a tip's state set as 1.0 or 0.0: a bit-field extract and an
 // integer-to-double conversion.  (Building the double's high word from a
 // mask needs a third instruction to copy the zero low word into each
@@ -403,7 +427,8 @@ __global__ __launch_bounds__(64 * C) void prune4_kernel(
     const Op* __restrict__ ops, const int* __restrict__ prog_begin, const uint8_t* __restrict__ tips,
     T* clv, int* scl, const double* __restrict__ P, const double* __restrict__ roots,
     const double* __restrict__ site_w, double* __restrict__ site_lnl, double* __restrict__ partial,
-    int m, int m_pad, int n_taxa, int n_main, int prog_main, int blk0, int prog_piece, int tail0, int n_tail) {
+    int m, int m_pad, int n_taxa, int n_main, int prog_main, int blk0, int prog_piece, int tail0, int n_tail,
+    unsigned* fold_counter, double* fold_out) {
     __shared__ int xsc[C][64];
     __shared__ double xroot[C][64];
     __shared__ double2 xb[C][2][64];
@@ -444,6 +469,7 @@ __global__ __launch_bounds__(64 * C) void prune4_kernel(
     const double w = live ? (site_w ? site_w[s] : 1.0) : 0.0;
     block_sum_store(live ? w * lnl : 0.0,
                     partial + (prog_main >= 0 ? (size_t)bx : (size_t)y * gridDim.x + blockIdx.x));
+    if (64 * C == kSumThreads && fold_counter) fold_total(partial, blk0 + n_main, fold_counter, fold_out);
 }
 
 // Batched scoring: many independent problems (and their NNI programs) in one
@@ -712,7 +738,8 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
     const Op* __restrict__ ops, const int* __restrict__ prog_begin, const uint32_t* __restrict__ tips, T* clv,
     int* scl, const double* __restrict__ Pm, const double* __restrict__ PTm, const double* __restrict__ roots,
     const double* __restrict__ site_w, double* __restrict__ site_lnl, double* __restrict__ partial, int m, int m_pad,
-    int S, uint32_t full, int n_taxa, int n_main, int prog_main, int blk0, int prog_piece, int tail0, int n_tail) {
+    int S, uint32_t full, int n_taxa, int n_main, int prog_main, int blk0, int prog_piece, int tail0, int n_tail,
+    unsigned* fold_counter, double* fold_out) {
     __shared__ int xsc[C][16 * G];
     __shared__ double xroot[C][16 * G];
     __shared__ uint32_t tl[TL ? kMfmaTipLdsTaxa * 16 * G : 1];
@@ -873,6 +900,7 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
         if (live) sum += (site_w ? site_w[s[q]] : 1.0) * lnl[q];
     }
     block_sum_store(sum, partial + (prog_main >= 0 ? (size_t)bx : (size_t)y * gridDim.x + blockIdx.x));
+    if (64 * C == kSumThreads && fold_counter) fold_total(partial, blk0 + n_main, fold_counter, fold_out);
 }
 
 // Compensated fixed-order sum of the per-workgroup partials of each program
@@ -1002,6 +1030,8 @@ struct phylo_ctx {
     bool tail_split = true;                               // phylo_set_tail_split
     int n_cu = 0;                                         // compute units of the device
     bool run_full = false;                                // the whole-tree launch stores every partial
+    bool folded = false;                                  // the last launch wrote the total itself (fold_total)
+    unsigned* d_fold = nullptr;                           // fold_total's counter (zero between launches)
 
     // device buffers
     void* d_tips = nullptr; size_t tips_bytes = 0;     // uint8 codes (S <= 4) or uint32 masks
@@ -1479,20 +1509,22 @@ void launch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
             hipLaunchKernelGGL((prune4_kernel<T, C, true>), dim3(main_b + K * tail), dim3(64 * C), 0, c->stream,
                                c->run_ops, c->d_prog, tips, clv, c->d_scl, c->d_P, c->d_roots, sw, site_lnl,
                                c->d_partial, c->m, c->m_pad, c->n_taxa, main_b, c->run_full ? 0 : 1, 0, base,
-                               main_b, tail);
+                               main_b, tail, nullptr, nullptr);
             // (the top program reads a few tips: no LDS staging)
+            unsigned* fold = 64 * C == kSumThreads ? c->d_fold : nullptr;
             hipLaunchKernelGGL((prune4_kernel<T, C, false>), dim3(tail), dim3(64 * C), 0, c->stream, c->run_ops,
                                c->d_prog, tips, clv, c->d_scl, c->d_P, c->d_roots, sw, site_lnl, c->d_partial, c->m,
-                               c->m_pad, c->n_taxa, tail, base + K, main_b, 0, 0, 1);
+                               c->m_pad, c->n_taxa, tail, base + K, main_b, 0, 0, 1, fold, c->h_total_dev);
+            c->folded = fold != nullptr;
         } else {
             hipLaunchKernelGGL((prune4_kernel<T, C, true>), grid, dim3(64 * C), 0, c->stream, c->run_ops, c->run_prog,
                                tips, clv, c->d_scl, c->d_P, c->d_roots, sw, site_lnl, c->d_partial, c->m, c->m_pad,
-                               c->n_taxa, (int)grid.x, -1, 0, 0, 0, 1);
+                               c->n_taxa, (int)grid.x, -1, 0, 0, 0, 1, nullptr, nullptr);
         }
     } else {
         hipLaunchKernelGGL((prune4_kernel<T, C, false>), grid, dim3(64 * C), 0, c->stream, c->run_ops, c->run_prog,
                            tips, clv, c->d_scl, c->d_P, c->d_roots, sw, site_lnl, c->d_partial, c->m, c->m_pad,
-                           c->n_taxa, (int)grid.x, -1, 0, 0, 0, 1);
+                           c->n_taxa, (int)grid.x, -1, 0, 0, 0, 1, nullptr, nullptr);
     }
 }
 
@@ -1527,22 +1559,25 @@ void launch_mfma(phylo_ctx* c, dim3 grid, double* site_lnl) {
             hipLaunchKernelGGL((prune_mfma_kernel<T, C, V, kMfmaGroups, true>), dim3(main_b + K * tail), dim3(64 * C),
                                0, c->stream, c->run_ops, c->d_prog, tips, clv, c->d_scl, c->d_Pm, c->d_PTm,
                                c->d_roots, sw, site_lnl, c->d_partial, c->m, c->m_pad, c->S, full, c->n_taxa, main_b,
-                               c->run_full ? 0 : 1, 0, base, main_b, tail);
+                               c->run_full ? 0 : 1, 0, base, main_b, tail, nullptr, nullptr);
             // (the top program reads a few tips: no LDS staging)
+            unsigned* fold = 64 * C == kSumThreads ? c->d_fold : nullptr;
             hipLaunchKernelGGL((prune_mfma_kernel<T, C, V, kMfmaGroups, false>), dim3(tail), dim3(64 * C), 0,
                                c->stream, c->run_ops, c->d_prog, tips, clv, c->d_scl, c->d_Pm, c->d_PTm, c->d_roots,
                                sw, site_lnl, c->d_partial, c->m, c->m_pad, c->S, full, c->n_taxa, tail, base + K,
-                               main_b, 0, 0, 1);
+                               main_b, 0, 0, 1, fold, c->h_total_dev);
+            c->folded = fold != nullptr;
         } else {
             hipLaunchKernelGGL((prune_mfma_kernel<T, C, V, kMfmaGroups, true>), grid, dim3(64 * C), 0, c->stream,
                                c->run_ops, c->run_prog, tips, clv, c->d_scl, c->d_Pm, c->d_PTm, c->d_roots, sw,
                                site_lnl, c->d_partial, c->m, c->m_pad, c->S, full, c->n_taxa, (int)grid.x, -1, 0, 0, 0,
-                               1);
+                               1, nullptr, nullptr);
         }
     } else {
         hipLaunchKernelGGL((prune_mfma_kernel<T, C, V, kMfmaGroups, false>), grid, dim3(64 * C), 0, c->stream,
                            c->run_ops, c->run_prog, tips, clv, c->d_scl, c->d_Pm, c->d_PTm, c->d_roots, sw, site_lnl,
-                           c->d_partial, c->m, c->m_pad, c->S, full, c->n_taxa, (int)grid.x, -1, 0, 0, 0, 1);
+                           c->d_partial, c->m, c->m_pad, c->S, full, c->n_taxa, (int)grid.x, -1, 0, 0, 0, 1, nullptr,
+                           nullptr);
     }
 }
 
@@ -1591,6 +1626,11 @@ int run_programs(phylo_ctx* c, const Op* d_ops, const int* d_prog, int n_prog, d
     }
     dim3 grid(nblk, n_prog);
     double* dsite = site_lnl_host ? c->d_sitelnl : nullptr;
+    if (!c->d_fold) {
+        HIPCHK(c, hipMalloc(&c->d_fold, sizeof(unsigned)));
+        HIPCHK(c, hipMemset(c->d_fold, 0, sizeof(unsigned)));
+    }
+    c->folded = false;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     int rc;
     if (c->S <= 4)
@@ -1600,9 +1640,13 @@ int run_programs(phylo_ctx* c, const Op* d_ops, const int* d_prog, int n_prog, d
     if (rc) return rc;
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    // the totals go straight into mapped pinned memory: no copy
-    hipLaunchKernelGGL(final_sum_kernel, dim3(n_prog), dim3(kSumThreads), 0, c->stream, c->d_partial, nblk, c->h_total_dev);
-    HIPCHK(c, hipGetLastError());
+    // the totals go straight into mapped pinned memory: no copy (a tail
+    // split's top launch has written the total already)
+    if (!c->folded) {
+        hipLaunchKernelGGL(final_sum_kernel, dim3(n_prog), dim3(kSumThreads), 0, c->stream, c->d_partial, nblk,
+                           c->h_total_dev);
+        HIPCHK(c, hipGetLastError());
+    }
     if (site_lnl_host)
         HIPCHK(c, hipMemcpyAsync(site_lnl_host, c->d_sitelnl, (size_t)n_prog * c->m * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1780,7 +1824,7 @@ void phylo_destroy(phylo_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* bufs[] = {c->d_tips, c->d_Pm, c->d_PTm, c->d_clv, c->d_scl, c->d_P, c->d_ops, c->d_prog, c->d_nni_ops, c->d_nni_prog,
                     c->d_roots, c->d_sitew,
-                    c->d_sitelnl, c->d_partial, c->d_eig, c->d_eff};
+                    c->d_sitelnl, c->d_partial, c->d_eig, c->d_eff, c->d_fold};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->h_total) (void)hipHostFree(c->h_total);
