@@ -136,9 +136,9 @@ def test_node_partials_follow_changed_inputs(oracle):
             np.testing.assert_allclose(got / scale, want / scale, atol=1e-10, err_msg=f"node {v}")
 
 
-@pytest.mark.parametrize("S,n,m", [(4, 128, 20000), (20, 64, 10000)])
+@pytest.mark.parametrize("S,n,m,C", [(4, 128, 20000, 4), (20, 64, 10000, 4), (4, 64, 20000, 8), (4, 64, 20000, 1)])
 @pytest.mark.parametrize("prec", [0, 1])
-def test_tail_split_matches_whole_tree_walk(oracle, S, n, m, prec):
+def test_tail_split_matches_whole_tree_walk(oracle, S, n, m, C, prec):
     """20,000 DNA sites are 313 blocks of 64 (10,000 20-state sites 313
     blocks of 32): past a full round over the GPU's CUs the tail blocks walk
     the tree as subtree pieces plus a short top program.  Per-site values
@@ -146,10 +146,12 @@ def test_tail_split_matches_whole_tree_walk(oracle, S, n, m, prec):
     against the whole-tree walk of the tail (phylo_set_tail_split(0)) FP64
     storage is bit-identical and FP32 storage differs only by the rounding of
     the extra stored partials.  Node partials and NNI scores, which read the
-    stored partials, follow the same bars."""
-    print(f"tail split S={S} n={n} m={m} prec={prec}", flush=True)
+    stored partials, follow the same bars.  With 4 rate categories the top
+    launch also forms the total (fold_total); otherwise the final-sum kernel
+    does."""
+    print(f"tail split S={S} n={n} m={m} C={C} prec={prec}", flush=True)
     kw = dict(rate_dist="gamma", freq_conc=2.0) if S > 4 else {}
-    wl = synth.gtr_gamma_workload(n_tips=n, m=m, S=S, C=4, alpha=0.5, gap_frac=0.05, seed=21 + S, **kw)
+    wl = synth.gtr_gamma_workload(n_tips=n, m=m, S=S, C=C, alpha=0.5, gap_frac=0.05, seed=21 + S + C, **kw)
     ref_tot, ref_site = oracle_problem(oracle, wl).loglik()
     t = wl.tree
     nodes = [t.root, t.left[t.root], t.n_tips + 3, t.n_nodes - 2]
@@ -160,7 +162,7 @@ def test_tail_split_matches_whole_tree_walk(oracle, S, n, m, prec):
                 ctx.set_tail_split(split)
                 ctx.set_keep_partials(keep)
                 tot, site = ctx.loglik(per_site=True)
-                parts = [ctx.node_partials(v, 4, S) for v in nodes]
+                parts = [ctx.node_partials(v, C, S) for v in nodes]
                 _, var = ctx.nni_score(1)
                 res[(split, keep)] = (tot, site, parts, var)
     for (split, keep), (tot, site, _, _) in res.items():
