@@ -154,7 +154,7 @@ def test_tail_split_matches_whole_tree_walk(oracle, S, n, m, C, prec):
     wl = synth.gtr_gamma_workload(n_tips=n, m=m, S=S, C=C, alpha=0.5, gap_frac=0.05, seed=21 + S + C, **kw)
     ref_tot, ref_site = oracle_problem(oracle, wl).loglik()
     t = wl.tree
-    nodes = [t.root, t.left[t.root], t.n_tips + 3, t.n_nodes - 2]
+    nodes = [v for v in (t.root, t.left[t.root], t.right[t.root], t.n_tips + 3, t.n_nodes - 2) if v >= t.n_tips]
     res = {}
     for split in (True, False):
         for keep in (False, True):
