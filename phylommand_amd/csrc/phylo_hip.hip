@@ -410,20 +410,11 @@ __device__ inline double combine_categories(double lc, int sc, int cat, int lane
     return log(lc) + sc * kLn2;
 }
 
-// prog_main value of a program-major plain launch (see prune4_kernel).
-constexpr int kProgMajor = -2;
-
 // Taxa whose tip codes a DNA workgroup stages in LDS (4 bits per site).
 constexpr int kTipLdsTaxa = 256;
 
-// Plain launch (prog_main = -1): grid = (m_pad / 64, n_programs); workgroup
+// Plain launch (prog_main < 0): grid = (m_pad / 64, n_programs); workgroup
 // (x, y) runs ops[prog_begin[y] .. prog_begin[y+1]) over site block x.
-// Program-major launch (prog_main = kProgMajor, n_main programs per
-// workgroup, n_tail programs in all): grid = (ceil(n_tail / n_main),
-// m_pad / 64), workgroup (x, y) runs programs x n_main ... over site block y
-// one after another, so the programs of one site block (an NNI
-// neighbourhood's variants, which read the same stored partials) run side by
-// side and share them in L2, in fewer, longer workgroups.
 // Split launch (prog_main >= 0, grid.y = 1; build_split_programs): workgroups
 // x < n_main run program prog_main over site block blk0 + x; the others run
 // piece program prog_piece + (x - n_main) / n_tail over site block
@@ -444,17 +435,10 @@ __global__ __launch_bounds__(64 * C) void prune4_kernel(
     __shared__ uint8_t tl[TL ? kTipLdsTaxa * 32 : 1];
     const int lane = threadIdx.x & 63;
     const int cat = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    // row: the program's output row (site values and block partials)
-    int bx = blockIdx.x, prog = blockIdx.y, row = blockIdx.y, nb = gridDim.x, n_run = 1;
+    const int y = blockIdx.y;
+    int bx = blockIdx.x, prog = y;
     bool has_root = true;
-    if (prog_main == kProgMajor) {
-        // n_main programs per workgroup (of n_tail in all), one after another
-        bx = blockIdx.y;
-        prog = row = (int)blockIdx.x * n_main;
-        nb = gridDim.y;
-        n_run = min(n_main, n_tail - prog);
-    } else if (prog_main >= 0) {
-        row = 0;
+    if (prog_main >= 0) {
         if ((int)blockIdx.x < n_main) {
             bx = blk0 + (int)blockIdx.x;
             prog = prog_main;
@@ -474,18 +458,17 @@ __global__ __launch_bounds__(64 * C) void prune4_kernel(
         }
         __syncthreads();
     }
+    double lc;
+    int sc;
+    walk4<T, C, TL>(ops, prog_begin[prog], prog_begin[prog + 1], tips, tl, clv, scl, P, roots, m_pad, s, lane, cat, xb,
+                    lc, sc);
+    if (!has_root) return;    // a piece: its stores are its result (uniform per workgroup)
+    const double lnl = combine_categories<C>(lc, sc, cat, lane, xroot, xsc);
     const bool live = s < m && cat == 0;
+    if (live && site_lnl) site_lnl[(size_t)y * m + s] = lnl;
     const double w = live ? (site_w ? site_w[s] : 1.0) : 0.0;
-    for (int it = 0; it < n_run; ++it, ++prog, ++row) {
-        double lc;
-        int sc;
-        walk4<T, C, TL>(ops, prog_begin[prog], prog_begin[prog + 1], tips, tl, clv, scl, P, roots, m_pad, s, lane, cat,
-                        xb, lc, sc);
-        if (!has_root) return;    // a piece: its stores are its result (uniform per workgroup)
-        const double lnl = combine_categories<C>(lc, sc, cat, lane, xroot, xsc);
-        if (live && site_lnl) site_lnl[(size_t)row * m + s] = lnl;
-        block_sum_store(live ? w * lnl : 0.0, partial + (size_t)row * nb + bx);
-    }
+    block_sum_store(live ? w * lnl : 0.0,
+                    partial + (prog_main >= 0 ? (size_t)bx : (size_t)y * gridDim.x + blockIdx.x));
     if (64 * C == kSumThreads && fold_counter) fold_total(partial, blk0 + n_main, fold_counter, fold_out);
 }
 
@@ -761,15 +744,10 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
     __shared__ double xroot[C][16 * G];
     __shared__ uint32_t tl[TL ? kMfmaTipLdsTaxa * 16 * G : 1];
     // work mapping as in prune4_kernel (plain or tail-split launch)
-    // row: the program's output row (site values and block partials)
-    int bx = blockIdx.x, prog = blockIdx.y, row = blockIdx.y, nb = gridDim.x;
+    const int y = blockIdx.y;
+    int bx = blockIdx.x, prog = y;
     bool has_root = true;
-    if (prog_main == kProgMajor) {
-        bx = blockIdx.y;
-        prog = row = blockIdx.x;
-        nb = gridDim.y;
-    } else if (prog_main >= 0) {
-        row = 0;
+    if (prog_main >= 0) {
         if ((int)blockIdx.x < n_main) {
             bx = blk0 + (int)blockIdx.x;
             prog = prog_main;
@@ -918,10 +896,10 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
 #pragma unroll
     for (int q = 0; q < G; ++q) {
         const bool live = s[q] < m && cat == 0 && g == 0;
-        if (live && site_lnl) site_lnl[(size_t)row * m + s[q]] = lnl[q];
+        if (live && site_lnl) site_lnl[(size_t)y * m + s[q]] = lnl[q];
         if (live) sum += (site_w ? site_w[s[q]] : 1.0) * lnl[q];
     }
-    block_sum_store(sum, partial + (size_t)row * nb + bx);
+    block_sum_store(sum, partial + (prog_main >= 0 ? (size_t)bx : (size_t)y * gridDim.x + blockIdx.x));
     if (64 * C == kSumThreads && fold_counter) fold_total(partial, blk0 + n_main, fold_counter, fold_out);
 }
 
@@ -1053,8 +1031,6 @@ struct phylo_ctx {
     int n_cu = 0;                                         // compute units of the device
     bool run_full = false;                                // the whole-tree launch stores every partial
     bool folded = false;                                  // the last launch wrote the total itself (fold_total)
-    bool run_prog_major = false;                          // the launch in flight is program-major
-    int run_vpw = 1, run_n_prog = 1;                      // its programs per workgroup, programs in all
     unsigned* d_fold = nullptr;                           // fold_total's counter (zero between launches)
 
     // device buffers
@@ -1543,16 +1519,12 @@ void launch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
         } else {
             hipLaunchKernelGGL((prune4_kernel<T, C, true>), grid, dim3(64 * C), 0, c->stream, c->run_ops, c->run_prog,
                                tips, clv, c->d_scl, c->d_P, c->d_roots, sw, site_lnl, c->d_partial, c->m, c->m_pad,
-                               c->n_taxa, c->run_prog_major ? c->run_vpw : (int)grid.x,
-                               c->run_prog_major ? kProgMajor : -1, 0, 0, 0, c->run_prog_major ? c->run_n_prog : 1,
-                               nullptr, nullptr);
+                               c->n_taxa, (int)grid.x, -1, 0, 0, 0, 1, nullptr, nullptr);
         }
     } else {
         hipLaunchKernelGGL((prune4_kernel<T, C, false>), grid, dim3(64 * C), 0, c->stream, c->run_ops, c->run_prog,
                            tips, clv, c->d_scl, c->d_P, c->d_roots, sw, site_lnl, c->d_partial, c->m, c->m_pad,
-                           c->n_taxa, c->run_prog_major ? c->run_vpw : (int)grid.x,
-                           c->run_prog_major ? kProgMajor : -1, 0, 0, 0, c->run_prog_major ? c->run_n_prog : 1,
-                           nullptr, nullptr);
+                           c->n_taxa, (int)grid.x, -1, 0, 0, 0, 1, nullptr, nullptr);
     }
 }
 
@@ -1567,13 +1539,6 @@ int dispatch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
     }
     return PHYLO_OK;
 }
-
-// Programs per workgroup of a program-major DNA launch (an NNI
-// neighbourhood): the variants of one site block run in turn in one
-// workgroup, sharing its setup and the block's stored partials (250-variant
-// neighbourhood of the 128-taxon tree: 1 per workgroup 2.40 ms, 4 2.26, 8
-// 2.34, 16 3.14).
-constexpr int kProgsPerWorkgroup = 4;
 
 // Site groups per wave of the MFMA walker: two independent MFMA chains per
 // wave share each P operand load and overlap their memory latency.
@@ -1605,14 +1570,13 @@ void launch_mfma(phylo_ctx* c, dim3 grid, double* site_lnl) {
         } else {
             hipLaunchKernelGGL((prune_mfma_kernel<T, C, V, kMfmaGroups, true>), grid, dim3(64 * C), 0, c->stream,
                                c->run_ops, c->run_prog, tips, clv, c->d_scl, c->d_Pm, c->d_PTm, c->d_roots, sw,
-                               site_lnl, c->d_partial, c->m, c->m_pad, c->S, full, c->n_taxa, (int)grid.x,
-                               c->run_prog_major ? kProgMajor : -1, 0, 0, 0, 1, nullptr, nullptr);
+                               site_lnl, c->d_partial, c->m, c->m_pad, c->S, full, c->n_taxa, (int)grid.x, -1, 0, 0, 0,
+                               1, nullptr, nullptr);
         }
     } else {
         hipLaunchKernelGGL((prune_mfma_kernel<T, C, V, kMfmaGroups, false>), grid, dim3(64 * C), 0, c->stream,
                            c->run_ops, c->run_prog, tips, clv, c->d_scl, c->d_Pm, c->d_PTm, c->d_roots, sw, site_lnl,
-                           c->d_partial, c->m, c->m_pad, c->S, full, c->n_taxa, (int)grid.x,
-                           c->run_prog_major ? kProgMajor : -1, 0, 0, 0, 1, nullptr,
+                           c->d_partial, c->m, c->m_pad, c->S, full, c->n_taxa, (int)grid.x, -1, 0, 0, 0, 1, nullptr,
                            nullptr);
     }
 }
@@ -1660,13 +1624,7 @@ int run_programs(phylo_ctx* c, const Op* d_ops, const int* d_prog, int n_prog, d
         HIPCHK(c, hipHostGetDevicePointer(reinterpret_cast<void**>(&c->h_total_dev), c->h_total, 0));
         c->h_total_bytes = (size_t)n_prog * sizeof(double);
     }
-    // several programs (an NNI neighbourhood): program-major, so that the
-    // programs of one site block run side by side and share its partials
-    // (the DNA walker runs kProgsPerWorkgroup programs per workgroup)
-    c->run_prog_major = n_prog > 1 && nblk <= 65535;
-    c->run_vpw = mfma ? 1 : kProgsPerWorkgroup;
-    c->run_n_prog = n_prog;
-    const dim3 grid = c->run_prog_major ? dim3((n_prog + c->run_vpw - 1) / c->run_vpw, nblk) : dim3(nblk, n_prog);
+    dim3 grid(nblk, n_prog);
     double* dsite = site_lnl_host ? c->d_sitelnl : nullptr;
     if (!c->d_fold) {
         HIPCHK(c, hipMalloc(&c->d_fold, sizeof(unsigned)));
@@ -2377,16 +2335,27 @@ void build_work_lists(phylo_batch* b, int bpi, const std::vector<int>& prog_len)
         b->seg1.push_back(out);
     }
     const int base2 = out;
+    // a variant's blocks keep consecutive partial slots (its segment), but
+    // the list runs block by block over the variants, so that items of equal
+    // cost that run side by side are variants of one site block and share
+    // its stored partials in L2
     for (size_t i = 0; i < b->inst.size(); ++i) {
         const auto& I = b->inst[i];
-        const int nb = I.m_pad / 64;
+        const int nb = I.m_pad / 64, n_items = (nb + bpi - 1) / bpi;
+        std::vector<int> first;
         for (int pg : I.nni_prog) {
             if (pg < 0) continue;
-            for (int blk = 0; blk < nb; blk += bpi) {
-                const int sites = std::min(bpi, nb - blk) * 64;
-                c2.push_back({(long)prog_len[pg] * sites, {(int)i, pg, blk, out++}});
-            }
+            first.push_back(out);
+            out += n_items;
             b->seg2.push_back(out - base2);
+        }
+        for (int blk = 0, q = 0; blk < nb; blk += bpi, ++q) {
+            const int sites = std::min(bpi, nb - blk) * 64;
+            size_t v = 0;
+            for (int pg : I.nni_prog) {
+                if (pg < 0) continue;
+                c2.push_back({(long)prog_len[pg] * sites, {(int)i, pg, blk, first[v++] + q}});
+            }
         }
     }
     auto by_cost = [](const Cand& x, const Cand& y) { return x.cost > y.cost; };
