@@ -1053,8 +1053,7 @@ struct phylo_ctx {
     double* d_sitew = nullptr; size_t sitew_bytes = 0;
     double* d_sitelnl = nullptr; size_t sitelnl_bytes = 0;
     double* d_partial = nullptr; size_t partial_bytes = 0;
-    double* d_eig = nullptr; size_t eig_bytes = 0;
-    double* d_eff = nullptr; size_t eff_bytes = 0;
+    double* d_eig = nullptr; size_t eig_bytes = 0;      // V, Vinv, lambda, category rates, edge lengths
     double* h_total = nullptr; size_t h_total_bytes = 0;  // pinned, mapped: final_sum_kernel writes here
     double* h_total_dev = nullptr;                        // its device address
     // Pinned staging for the per-evaluation uploads (eigen system + edge
@@ -1385,10 +1384,11 @@ int upload_pmatrices(phylo_ctx* c) {
         w = std::copy(c->lam.begin(), c->lam.end(), w);
         w = std::copy(c->cat_rate.begin(), c->cat_rate.end(), w);
         std::copy(c->eff.begin(), c->eff.end(), w);
-        HIPCHK(c, ensure(c->d_eig, c->eig_bytes, n_eig * sizeof(double)));
-        HIPCHK(c, ensure(c->d_eff, c->eff_bytes, c->eff.size() * sizeof(double)));
-        HIPCHK(c, hipMemcpyAsync(c->d_eig, h, n_eig * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->d_eff, h + n_eig, c->eff.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        // eigen system and edge lengths in one buffer: one copy per model or length change
+        HIPCHK(c, ensure(c->d_eig, c->eig_bytes, (n_eig + c->eff.size()) * sizeof(double)));
+        HIPCHK(c, hipMemcpyAsync(c->d_eig, h, (n_eig + c->eff.size()) * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+        const double* d_eff = c->d_eig + n_eig;
         HIPCHK(c, stage_release(c->stage_P, c->stream));
         const double* dV = c->d_eig;
         const double* dVi = dV + (size_t)S * S;
@@ -1397,7 +1397,7 @@ int upload_pmatrices(phylo_ctx* c) {
         const int threads = 256;
         const long blocks = ((long)nP + threads - 1) / threads;
         hipLaunchKernelGGL(pbuild_eigen_kernel, dim3((unsigned)blocks), dim3(threads), 0, c->stream, dV, dVi, dl,
-                           c->d_eff, dr, S, Sp, c->C, c->n_nodes, c->d_P);
+                           d_eff, dr, S, Sp, c->C, c->n_nodes, c->d_P);
         HIPCHK(c, hipGetLastError());
     } else {
         std::vector<double> P;
@@ -1824,7 +1824,7 @@ void phylo_destroy(phylo_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* bufs[] = {c->d_tips, c->d_Pm, c->d_PTm, c->d_clv, c->d_scl, c->d_P, c->d_ops, c->d_prog, c->d_nni_ops, c->d_nni_prog,
                     c->d_roots, c->d_sitew,
-                    c->d_sitelnl, c->d_partial, c->d_eig, c->d_eff, c->d_fold};
+                    c->d_sitelnl, c->d_partial, c->d_eig, c->d_fold};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->h_total) (void)hipHostFree(c->h_total);
