@@ -419,7 +419,9 @@ constexpr int kTipLdsTaxa = 256;
 // x < n_main run program prog_main over site block blk0 + x; the others run
 // piece program prog_piece + (x - n_main) / n_tail over site block
 // tail0 + (x - n_main) % n_tail, which has no root: they store their results
-// and leave.
+// and leave.  The split's second launch (the tail's top program) passes
+// fold_counter: its last workgroup also writes the evaluation's total
+// (fold_total) when the workgroup has kSumThreads threads.
 // TL (n_taxa <= kTipLdsTaxa): the workgroup first copies its 64 sites of
 // every tip into LDS, so no op of the walk waits on a global tip load.
 template <typename T, int C, bool TL>
