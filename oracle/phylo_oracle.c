@@ -27,9 +27,16 @@
  *   orc_node_partials       golden [&P={...}] labels from
  *                           draw_normalized_likelihood_on_nodes
  *   orc_nni_reference       golden treebender --nni all tree lists
- *   orc_simulate            parity unpinned (the reference draws from libc
- *                           rand() seeded by time(NULL), tree.h:61-64);
- *                           checked only by distribution tests
+ *   orc_nni_list mode 0     the same golden lists (variant order included)
+ *   orc_nni_list mode 1     brute force on split sets: the variants are the
+ *                           2(n-3) distinct unrooted trees at Robinson-Foulds
+ *                           distance 2, for every topology with n = 5..8
+ *   orc_simulate            the reference's draws cannot be replayed (libc
+ *                           rand() seeded by time(NULL), tree.h:61-64); its
+ *                           distribution is pinned: chi-square of 4-taxon
+ *                           pattern counts against exp(site lnL), with
+ *                           P = I, swapped lengths, ignored categories and
+ *                           wrong root frequencies all rejected
  */
 #include <ctype.h>
 #include <math.h>

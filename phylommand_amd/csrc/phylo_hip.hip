@@ -1792,6 +1792,29 @@ bool append_nni_program(const TV* c, const std::vector<int>& par, int v, int sid
     return true;
 }
 
+// Checks a rooted topology over n_taxa tips (nodes 0..n_taxa-1, no children)
+// and internal nodes n_taxa..n_nodes-1: one or two valid children each, every
+// node but the root with exactly one parent.  Returns the reason, or nullptr.
+const char* check_topology(int n_taxa, int n_nodes, int root, const int32_t* left, const int32_t* right) {
+    if (n_nodes <= n_taxa || root < n_taxa || root >= n_nodes || !left || !right) return "bad node count or root";
+    std::vector<int> indeg(n_nodes, 0);
+    for (int v = 0; v < n_nodes; ++v) {
+        const bool tip = v < n_taxa;
+        if (tip && (left[v] != -1 || right[v] != -1)) return "tips must have no children";
+        if (!tip) {
+            if (left[v] < 0 || left[v] >= n_nodes || right[v] < -1 || right[v] >= n_nodes || left[v] == right[v] ||
+                left[v] == v || right[v] == v)
+                return "bad child index";
+            ++indeg[left[v]];
+            if (right[v] >= 0) ++indeg[right[v]];
+        }
+    }
+    for (int v = 0; v < n_nodes; ++v)
+        if ((v == root && indeg[v] != 0) || (v != root && indeg[v] != 1))
+            return "every non-root node needs exactly one parent";
+    return nullptr;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1863,6 +1886,12 @@ int phylo_set_tips(phylo_ctx* c, int n_taxa, int n_sites, int n_states, const ui
             snprintf(buf, sizeof buf, "taxon %zu lacks a character state for character %zu", i / n_sites, i % n_sites);
             return fail(c, PHYLO_ERR_DATA, buf);
         }
+    if (c->have_topo && n_taxa != c->n_taxa) {
+        // node numbering (tips 0..n_taxa-1) changed: the topology and the
+        // lengths set for the old numbering no longer apply
+        c->have_topo = false;
+        c->have_eff = false;
+    }
     c->n_taxa = n_taxa;
     c->m = n_sites;
     c->S = n_states;
@@ -1882,7 +1911,12 @@ int phylo_set_tips(phylo_ctx* c, int n_taxa, int n_sites, int n_states, const ui
 int phylo_set_site_weights(phylo_ctx* c, const double* w) {
     if (!c) return PHYLO_ERR_ARG;
     if (!c->have_tips) return fail(c, PHYLO_ERR_STATE, "set_site_weights before set_tips");
+    // the stored partials do not depend on the weights, but the cached total
+    // of the last evaluation (the NNI base and no-op variants) does
+    c->clv_valid = false;
     if (!w) { c->have_site_w = false; c->dirty_site_w = false; return PHYLO_OK; }
+    for (int s = 0; s < c->m; ++s)
+        if (!std::isfinite(w[s])) return fail(c, PHYLO_ERR_ARG, "set_site_weights: non-finite weight");
     c->site_w.assign(w, w + c->m);
     c->have_site_w = true;
     c->dirty_site_w = true;
@@ -1892,23 +1926,8 @@ int phylo_set_site_weights(phylo_ctx* c, const double* w) {
 int phylo_set_topology(phylo_ctx* c, int n_nodes, int root, const int32_t* left, const int32_t* right) {
     if (!c) return PHYLO_ERR_ARG;
     if (!c->have_tips) return fail(c, PHYLO_ERR_STATE, "set_topology before set_tips");
-    if (n_nodes <= c->n_taxa || root < c->n_taxa || root >= n_nodes || !left || !right)
-        return fail(c, PHYLO_ERR_ARG, "set_topology: bad node count or root");
-    std::vector<int> indeg(n_nodes, 0);
-    for (int v = 0; v < n_nodes; ++v) {
-        const bool tip = v < c->n_taxa;
-        if (tip && (left[v] != -1 || right[v] != -1)) return fail(c, PHYLO_ERR_ARG, "set_topology: tips must have no children");
-        if (!tip) {
-            if (left[v] < 0 || left[v] >= n_nodes || right[v] < -1 || right[v] >= n_nodes || left[v] == right[v] ||
-                left[v] == v || right[v] == v)
-                return fail(c, PHYLO_ERR_ARG, "set_topology: bad child index");
-            ++indeg[left[v]];
-            if (right[v] >= 0) ++indeg[right[v]];
-        }
-    }
-    for (int v = 0; v < n_nodes; ++v)
-        if ((v == root && indeg[v] != 0) || (v != root && indeg[v] != 1))
-            return fail(c, PHYLO_ERR_ARG, "set_topology: every non-root node needs exactly one parent");
+    const char* why = check_topology(c->n_taxa, n_nodes, root, left, right);
+    if (why) return fail(c, PHYLO_ERR_ARG, std::string("set_topology: ") + why);
     c->n_nodes = n_nodes;
     c->root = root;
     c->left.assign(left, left + n_nodes);
@@ -2075,7 +2094,12 @@ int phylo_get_pmatrix(phylo_ctx* c, int node, double* P) {
     }
     const int Sp = padded_states(c->S_model), S = c->S_model;
     std::vector<double> buf((size_t)c->C * Sp * Sp);
-    HIPCHK(c, hipMemcpy(buf.data(), c->d_P + (size_t)node * c->C * Sp * Sp, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+    // the P build runs on the context's non-blocking stream: copy on that
+    // stream, after it, not on the null stream
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(buf.data(), c->d_P + (size_t)node * c->C * Sp * Sp, buf.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int k = 0; k < c->C; ++k)
         for (int i = 0; i < S; ++i)
             for (int j = 0; j < S; ++j) P[((size_t)k * S + i) * S + j] = buf[((size_t)k * Sp + i) * Sp + j];

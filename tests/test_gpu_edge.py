@@ -161,6 +161,52 @@ def test_site_weights_give_the_weighted_total(oracle):
     np.testing.assert_allclose(site, ref_site, rtol=1e-5)
 
 
+def test_site_weights_reach_the_nni_base(oracle):
+    """Weights set after an evaluation that kept every partial: the NNI
+    base, the no-op variants and every scored variant use the weights."""
+    print("site weights then nni_score: 14 taxa x 400 sites", flush=True)
+    wl = synth.gtr_gamma_workload(n_tips=14, m=400, C=4, alpha=0.6, gap_frac=0.05, seed=13)
+    t = wl.tree
+    w = np.random.default_rng(4).integers(1, 6, 400).astype(float)
+    mb = np.zeros((t.n_nodes, 400), np.uint32)
+    mb[:14] = wl.masks
+    with gpu_context(wl) as ctx:
+        ctx.set_keep_partials(True)
+        unweighted = ctx.loglik()
+        ctx.set_site_weights(w)
+        base, variants = ctx.nni_score(1)
+        weighted = ctx.loglik()
+    _, ref_site = oracle_problem(oracle, wl).loglik()
+    assert abs(weighted - float(np.dot(w, ref_site))) <= 1e-3
+    assert abs(base - weighted) <= 1e-9 * abs(weighted)
+    assert abs(base - unweighted) > 1.0
+    for v, side, lnl in variants[:6]:
+        l2, r2 = oracle.nni_apply(t.left, t.right, t.root, v, side)
+        _, site2 = oracle.Problem(l2, r2, t.root, t.blen, mb, 4, wl.Q, wl.pi, pi=wl.pi, cat_rates=wl.cat_rates,
+                                  cat_w=wl.cat_weights).loglik()
+        assert abs(lnl - float(np.dot(w, site2))) <= 1e-3, (v, side)
+
+
+@pytest.mark.parametrize("S,C", [(4, 4), (20, 2)])
+def test_eigen_pmatrix_follows_new_lengths(oracle, S, C):
+    """P built on the device by the eigen builder, read back after the edge
+    lengths change (the build runs on the context's own stream)."""
+    print(f"eigen pmatrix after new lengths S={S} C={C}", flush=True)
+    kw = dict(rate_dist="gamma", freq_conc=2.0) if S > 4 else {}
+    wl = synth.gtr_gamma_workload(n_tips=12, m=100, S=S, C=C, alpha=0.8, gap_frac=0.0, seed=21, **kw)
+    t = wl.tree
+    with gpu_context(wl) as ctx:
+        ctx.loglik()
+        for scale in (1.0, 2.5, 0.3):
+            eff = t.blen * scale
+            ctx.set_edge_lengths(eff)
+            for v in (0, 5, t.n_tips + 1):
+                got = ctx.pmatrix(v, C, S)
+                for k in range(C):
+                    want = oracle.pmatrix(wl.Q, eff[v] * wl.cat_rates[k], oracle.EIG, wl.pi)
+                    np.testing.assert_allclose(got[k], want, rtol=1e-9, atol=1e-11)
+
+
 @pytest.mark.parametrize("S,C", [(5, 1), (17, 2), (19, 4), (20, 8), (21, 2), (32, 4)])
 def test_wide_alphabet_limits(oracle, S, C):
     wl = synth.gtr_gamma_workload(n_tips=10, m=130, S=S, C=C, alpha=0.8, gap_frac=0.05, seed=S,
@@ -196,6 +242,9 @@ def test_invalid_inputs_are_rejected():
         zero[0, 0] = 0
         with pytest.raises(P.PhyloError):
             ctx.set_tips(zero, 4)                  # a tip with no state
+        ctx.set_tips(wl.masks[:5], 4)              # fewer taxa: the 6-taxon topology no longer fits
+        with pytest.raises(P.PhyloError):
+            ctx.loglik()
 
 
 def balanced(depth, blen):

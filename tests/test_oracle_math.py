@@ -231,13 +231,3 @@ def test_planted_mistakes_fail(oracle):
     Q_nopi = synth.gtr_Q(np.ones(6), np.full(4, 0.25))
     assert np.abs(oracle.pmatrix(Q_nopi, 0.3, oracle.EIG, np.full(4, .25)) - P).max() > 1e-3
 
-
-def test_simulator_distribution(oracle):
-    """orc_simulate (parity unpinned: the reference draws from time-seeded
-    rand()): tip state frequencies of a long simulation match pi."""
-    rng = np.random.default_rng(22)
-    t = synth.yule_tree(4, rng, 0.5)
-    Q, pi = rand_gtr(rng)
-    st = oracle.simulate(t.left, t.right, t.root, t.blen, 20000, Q, pi, pi, seed=5)
-    freq = np.bincount(st[0], minlength=4) / 20000
-    assert np.abs(freq - pi).max() < 0.02
