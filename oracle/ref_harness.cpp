@@ -14,6 +14,8 @@
 //            multi-site total with a rate_model (treeator.cpp:919, :923)
 //   pmatrix  S TR SPEC PARAMS FREQS T1 [T2 ...]     sub_model::set_P_matrix dumps
 //   tmbl     TIMEARG B1 D1 [B2 D2 ...]              get_time_mod_branch_length dumps
+//   ratearg  clade|time ARG                        the rate_model pars_clade_rates_arg /
+//            pars_rates_and_times_arg build, through its getters
 //   nodes    TREE FASTA ALPHA TR SPEC PARAMS FREQS NSITES
 //            draw_normalized_likelihood_on_nodes + print_newick
 //   parse    TREE                                   print_newick of each parsed tree
@@ -199,6 +201,23 @@ int main(int argc, char** argv) {
             for (unsigned int i = 0; i < S; ++i)
                 for (unsigned int j = 0; j < S; ++j) printf("%.17g%c", model.get_P_value(i, j), j + 1 < S ? ' ' : '\n');
         }
+        return 0;
+    }
+    if (mode == "ratearg") {
+        // the model a -V (clade) or time-rate argument builds, through the getters
+        rate_model rate_mod;
+        const std::string kind = argv[2];
+        std::vector<std::string> sets;
+        const bool ok = kind == "clade" ? rate_mod.pars_clade_rates_arg(argv[3], sets)
+                                        : rate_mod.pars_rates_and_times_arg(argv[3]);
+        printf("ok %d\n", ok ? 1 : 0);
+        printf("time_rates %u cut_offs %u\n", rate_mod.get_n_time_rates(), rate_mod.get_n_cut_offs());
+        for (unsigned int i = 0; i < rate_mod.get_n_cut_offs(); ++i)
+            printf("cut %.17g rate %.17g\n", (double)rate_mod.get_cut_off(i), (double)rate_mod.get_rate_in_time(i));
+        printf("clade_rates %u clades %u\n", rate_mod.get_n_clade_rates(), rate_mod.get_n_clades());
+        for (unsigned int i = 0; i < rate_mod.get_n_clades(); ++i)
+            printf("clade %u rate %.17g\n", i, (double)rate_mod.get_rate_clade(i));
+        for (size_t i = 0; i < sets.size(); ++i) printf("set %zu [%s]\n", sets[i].size(), sets[i].c_str());
         return 0;
     }
     if (mode == "tmbl") {

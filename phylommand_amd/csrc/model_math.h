@@ -2,6 +2,14 @@
 // symmetric eigen-decomposition (Householder tridiagonalisation + implicit QL),
 // P(t) by eigen-system, by scaling-and-squaring Pade, and by the reference's
 // (I + Q t / 2^20)^(2^20) squaring (marth.cpp:129-140, sub_model.cpp:115-127).
+//
+// Published methods followed: the symmetric eigen-solver is the classical
+// Householder tridiagonalisation with accumulated transforms followed by the
+// QL algorithm with implicit shifts, as in EISPACK's TRED2 and TQL2 (Martin,
+// Reinsch and Wilkinson, Numer. Math. 11 (1968) 181-195; Bowdler, Martin,
+// Reinsch and Wilkinson, Numer. Math. 11 (1968) 293-306).  The Pade form is
+// the degree-13 scaling-and-squaring method of Higham, SIAM J. Matrix Anal.
+// Appl. 26 (2005) 1179-1193.
 #pragma once
 
 #include <cmath>

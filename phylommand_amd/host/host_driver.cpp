@@ -9,6 +9,7 @@
 //   nniscore TREE FASTA ALPHA TR SPEC PARAMS FREQS MODE     NNI neighbourhood lnL
 //   parse    TREE          print_newick of every tree in the file
 //   tmbl     TIMEARG B D [B D ...]
+//   ratearg  clade|time ARG   the rate model a -V / time-rate argument builds
 //   pmatrix  S TR SPEC PARAMS FREQS T [T ...]
 //   simulate TREE ALPHA TR SPEC PARAMS FREQS NCHAR SEED    fasta on stdout
 // ALPHA is dna | protein | binary | an alphabet file; lists are comma
@@ -168,6 +169,23 @@ int main(int argc, char** argv) {
             for (auto& x : out) x.print_newick();
             break;
         }
+        return 0;
+    }
+    if (mode == "ratearg") {
+        // the model a -V (clade) or time-rate argument builds, through the getters
+        rate_model rate_mod;
+        const std::string kind = argv[2];
+        std::vector<std::string> sets;
+        const bool ok = kind == "clade" ? rate_mod.pars_clade_rates_arg(argv[3], sets)
+                                        : rate_mod.pars_rates_and_times_arg(argv[3]);
+        printf("ok %d\n", ok ? 1 : 0);
+        printf("time_rates %u cut_offs %u\n", rate_mod.get_n_time_rates(), rate_mod.get_n_cut_offs());
+        for (unsigned int i = 0; i < rate_mod.get_n_cut_offs(); ++i)
+            printf("cut %.17g rate %.17g\n", (double)rate_mod.get_cut_off(i), (double)rate_mod.get_rate_in_time(i));
+        printf("clade_rates %u clades %u\n", rate_mod.get_n_clade_rates(), rate_mod.get_n_clades());
+        for (unsigned int i = 0; i < rate_mod.get_n_clades(); ++i)
+            printf("clade %u rate %.17g\n", i, (double)rate_mod.get_rate_clade(i));
+        for (size_t i = 0; i < sets.size(); ++i) printf("set %zu [%s]\n", sets[i].size(), sets[i].c_str());
         return 0;
     }
     if (mode == "tmbl") {

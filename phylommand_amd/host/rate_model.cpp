@@ -2,6 +2,8 @@
 #include "rate_model.h"
 
 #include <cstdlib>
+#include <string>
+#include <vector>
 
 namespace phylo_amd {
 
@@ -68,43 +70,77 @@ double rate_model::get_time_mod_branch_length(const double branch_length, const 
     return len;
 }
 
-bool rate_model::pars_clade_rates_arg(const char* argument, std::vector<std::string>& taxon_sets) {
-    unsigned int rate_class = 0;
-    std::string temp;
-    for (unsigned int j = 0; argument[j] != '\0'; ++j) {
-        if (argument[j] == ':') {
-            rate_class = get_n_clade_rates();
-            set_rate(rate_class, (float)atof(temp.c_str()));
-            temp.clear();
-        } else if (argument[j] == ';' || argument[j + 1] == '\0') {
-            if (argument[j + 1] == '\0') temp += argument[j];
-            taxon_sets.push_back(temp);
-            if (!set_clade_rate((unsigned int)taxon_sets.size() - 1, rate_class)) {
-                set_rate(rate_class, 1.0f);
-                if (!set_clade_rate((unsigned int)taxon_sets.size() - 1, rate_class)) return false;
-            }
-            temp.clear();
-        } else {
-            temp += argument[j];
+// ---- argument grammar -----------------------------------------------------
+//
+// Both arguments are lists of tokens.  A token ends at ':' (it is then a rate
+// and opens a new rate class), at the list separator (';' between taxon sets,
+// ',' between cut-off times) or at the end of the argument.  At the end the
+// final character stays in the last token unless it is ':', so a trailing
+// separator becomes part of the last item ("2:a,b;" names the taxon set
+// "a,b;"), which is how treeator's -V and time arguments are read
+// (rate_model.cpp:129-176).  Numbers are read with atof and stored as float.
+//
+//   clade argument:  (RATE ':' | TAXA (';' | end))*   TAXA: a taxon set, kept as text
+//   time argument:   (RATE ':' | TIME (',' | end))*
+//
+// Items after a rate belong to that rate's class; items before the first
+// rate belong to class 0, which a taxon set creates with rate 1 if no clade
+// rate exists yet, while a cut-off time there is an error.
+
+namespace {
+
+struct arg_token {
+    std::string text;
+    char end;        // ':', the list separator, or 0 at the end of the argument
+};
+
+std::vector<arg_token> split_rate_arg(const char* argument, char separator) {
+    std::vector<arg_token> tokens;
+    const std::string arg(argument ? argument : "");
+    size_t start = 0;
+    for (size_t i = 0; i < arg.size(); ++i) {
+        const char ch = arg[i];
+        if (ch == ':') {
+            tokens.push_back({arg.substr(start, i - start), ':'});
+            start = i + 1;
+        } else if (i + 1 == arg.size()) {
+            tokens.push_back({arg.substr(start), 0});
+            start = arg.size();
+        } else if (ch == separator) {
+            tokens.push_back({arg.substr(start, i - start), separator});
+            start = i + 1;
         }
+    }
+    return tokens;
+}
+
+float arg_number(const std::string& text) { return (float)std::atof(text.c_str()); }
+
+}  // namespace
+
+bool rate_model::pars_clade_rates_arg(const char* argument, std::vector<std::string>& taxon_sets) {
+    unsigned int current = 0;                       // rate class of the taxon sets that follow
+    for (const arg_token& tok : split_rate_arg(argument, ';')) {
+        if (tok.end == ':') {
+            current = get_n_clade_rates();
+            set_rate(current, arg_number(tok.text));
+            continue;
+        }
+        taxon_sets.push_back(tok.text);
+        if (current >= rates.size()) set_rate(current, 1.0f);    // a taxon set before any rate
+        if (!set_clade_rate((unsigned int)taxon_sets.size() - 1, current)) return false;
     }
     return get_n_clade_rates() > 0;
 }
 
 bool rate_model::pars_rates_and_times_arg(const char* argument) {
-    std::string temp;
-    unsigned int rate_class = 0;
-    for (unsigned int i = 0; argument[i] != '\0'; ++i) {
-        if (argument[i] == ':') {
-            rate_class = get_n_time_rates();
-            set_time_rate(rate_class, (float)atof(temp.c_str()));
-            temp.clear();
-        } else if (argument[i] == ',' || argument[i + 1] == '\0') {
-            if (argument[i + 1] == '\0') temp += argument[i];
-            if (!add_rate_for_time(rate_class, (float)atof(temp.c_str()))) return false;
-            temp.clear();
-        } else {
-            temp += argument[i];
+    unsigned int current = 0;                       // rate class of the cut-offs that follow
+    for (const arg_token& tok : split_rate_arg(argument, ',')) {
+        if (tok.end == ':') {
+            current = get_n_time_rates();
+            set_time_rate(current, arg_number(tok.text));
+        } else if (!add_rate_for_time(current, arg_number(tok.text))) {
+            return false;                           // a time before any rate
         }
     }
     return get_n_time_rates() > 0;

@@ -195,6 +195,19 @@ def main():
         tm.append({"arg": arg, "pairs": pairs, "out": [float(x) for x in out.split()]})
     write(os.path.join(HERE, "time_mod_length.json"), json.dumps(tm) + "\n")
 
+    # the rate models that -V (clade) and time-rate arguments build, edge cases
+    # of their token grammar included (rate_model.cpp:129-176)
+    ra = []
+    for kind, args in (("clade", ["2:a,b", "2:a,b;", "2:a;b;0.5:c", "a,b", "a,b;1.5:c", ":", "2:", ";", ";;",
+                                  "2:;", "1:a:2:b", "", "x", "0.5:a;;b", "3:abc;2:", "1e-2:t1,t2;4:t3",
+                                  "0.333333333333:x", "abc:def"]),
+                       ("time", ["2:0.1", "2:0.1,", "2:0.1,0.3,1:0.2", "0.1", "1:", ":", "2:0.5,0.1",
+                                 "1:0.2,0.2,2:0.2", "", "3:0.4;x", "0.5:1,2:0.25,0.75", ",", "2:,", "x:0.5",
+                                 "1.5:0.1,2.5:0.05,0.3"])):
+        for a in args:
+            ra.append({"kind": kind, "arg": a, "out": run("ratearg", kind, a)})
+    write(os.path.join(HERE, "rate_args.json"), json.dumps(ra, indent=1) + "\n")
+
     # whole-tree likelihoods with rate models (time cut-offs, clade rates)
     rng = np.random.default_rng(19)
     t = synth.yule_tree(8, rng, 0.15)

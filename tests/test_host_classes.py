@@ -53,3 +53,13 @@ def test_pmatrix_bit_exact():
         vals = np.array([[float(x) for x in line.split()] for line in out.strip().split("\n")])
         want = np.array(c["P"]).reshape(-1, c["S"])
         np.testing.assert_array_equal(vals, want, err_msg=key)
+
+
+def test_rate_arguments_build_the_reference_models():
+    """-V clade-rate and time-rate arguments, edge cases of their grammar
+    included (trailing separators, empty items, items before any rate),
+    against the models the compiled reference builds from them."""
+    cases = json.load(open(os.path.join(GOLDEN, "rate_args.json")))
+    assert len(cases) >= 30
+    for c in cases:
+        assert drive("ratearg", c["kind"], c["arg"]) == c["out"], (c["kind"], c["arg"])
