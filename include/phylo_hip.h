@@ -167,6 +167,29 @@ int phylo_synchronize(phylo_ctx *ctx);
  * kernel in the last phylo_loglik / phylo_nni_score call. */
 int phylo_last_kernel_ms(phylo_ctx *ctx, double *ms);
 
+/* One whole-tree evaluation that allocates nothing: no hipMalloc, hipFree,
+ * pinned-host or other device allocation on any call.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid it).  The
+ * inputs are those of phylo_set_tips, phylo_set_topology,
+ * phylo_set_edge_lengths, phylo_set_model (PHYLO_P_EIGEN, so Q must be
+ * reversible with respect to pi) and phylo_set_categories, for the DNA walker
+ * (1 to 4 states; 1, 2, 4 or 8 categories) with FP32 partials; the outputs are
+ * those of phylo_loglik.  The result equals phylo_loglik's on the same inputs
+ * with phylo_set_tail_split(ctx, 0) bit for bit.  The context supplies only
+ * the device, its stream and the error text (phylo_last_error); its own
+ * buffers are not touched.  All device memory comes from one 64 MiB array per
+ * GPU, held from the first call on that GPU until the process ends; a problem
+ * that needs more (tips, the partials the walk stores, P, sums: about 1 KiB
+ * per site at 128 taxa and 4 categories) is rejected with PHYLO_ERR_ARG.
+ * Every call on a GPU uses the same array, so calls that overlap on one GPU
+ * (from several threads or contexts) race; calls from one thread do not,
+ * since each returns only after its work is done. */
+int phylo_loglik_noalloc(phylo_ctx *ctx, int n_taxa, int n_sites, int n_states, const uint32_t *masks, int n_nodes,
+                         int root, const int32_t *left, const int32_t *right, const double *eff_len, const double *Q,
+                         const double *root_weights, const double *pi, int n_cat, const double *cat_rates,
+                         const double *cat_weights, double *site_lnl, double *total);
+
 /* ---- Batched scoring ------------------------------------------------------
  * Many independent DNA problems (S <= 4) of uneven taxon and site counts,
  * each optionally with its whole NNI neighbourhood, scored by two persistent

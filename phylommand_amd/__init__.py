@@ -65,6 +65,9 @@ SIGNATURES = [
     ("phylo_batch_destroy", None, [_c_void_p]),
     ("phylo_last_kernel_ms", _c_int, [_c_void_p, _c_dbl_p]),
     ("phylo_simulate", _c_int, [_c_void_p, _c_int, _c_dbl_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint8)]),
+    ("phylo_loglik_noalloc", _c_int, [_c_void_p, _c_int, _c_int, _c_int, ctypes.POINTER(ctypes.c_uint32), _c_int,
+                                      _c_int, _c_int_p, _c_int_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_int,
+                                      _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]),
     # include/phylo_learn.h
     ("phylo_sim_features", _c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int_p, _c_int_p, _c_int_p, _c_dbl_p,
                                     _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.c_double, ctypes.c_uint64, _c_flt_p,
@@ -301,6 +304,29 @@ class Context:
         self._check(self.lib.phylo_nni_score(self.h, mode, n, _dp(lnl), _ip(nodes), _ip(sides), ctypes.byref(nv),
                                              _dp(base)))
         return float(base[0]), list(zip(nodes[:n].tolist(), sides[:n].tolist(), lnl[:n].tolist()))
+
+    def loglik_noalloc(self, masks, left, right, root, eff, Q, root_weights, pi, cat_rates, cat_weights=None,
+                       per_site: bool = False):
+        """One evaluation through phylo_loglik_noalloc: no device allocation,
+        the workspace is the library's fixed scratch array; the context gives
+        only its device and stream."""
+        masks = np.ascontiguousarray(masks, dtype=np.uint32)
+        n_taxa, m = masks.shape
+        left = np.ascontiguousarray(left, dtype=np.int32)
+        right = np.ascontiguousarray(right, dtype=np.int32)
+        eff = np.ascontiguousarray(eff, dtype=np.float64)
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        rw = np.ascontiguousarray(root_weights, dtype=np.float64)
+        pi = np.ascontiguousarray(pi, dtype=np.float64)
+        cr = np.ascontiguousarray(cat_rates, dtype=np.float64)
+        cw = np.full(len(cr), 1.0 / len(cr)) if cat_weights is None else np.ascontiguousarray(cat_weights, np.float64)
+        tot = np.zeros(1)
+        site = np.zeros(m) if per_site else None
+        self._check(self.lib.phylo_loglik_noalloc(
+            self.h, n_taxa, m, Q.shape[0], masks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(left),
+            int(root), _ip(left), _ip(right), _dp(eff), _dp(Q), _dp(rw), _dp(pi), len(cr), _dp(cr), _dp(cw),
+            _dp(site) if per_site else None, _dp(tot)))
+        return (float(tot[0]), site) if per_site else float(tot[0])
 
     def nni_ops(self) -> int:
         n = ctypes.c_long(0)

@@ -31,6 +31,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2229,6 +2230,220 @@ int phylo_last_kernel_ms(phylo_ctx* c, double* ms) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free evaluation (phylo_loglik_noalloc)
+// ---------------------------------------------------------------------------
+
+namespace {
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr int kScratchDevices = 64;
+}  // namespace
+
+// The whole device workspace of phylo_loglik_noalloc: inputs staged from the
+// host, P matrices, stored partials and scalers, block sums and outputs.  One
+// copy per GPU, held until the process ends.  On the host this name is a
+// stand-in, not a device address: host code uses the per-device address
+// from hipGetSymbolAddress (scratch_base) and never the name itself.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_base[kScratchDevices] = {};
+
+// Device address of `scratch` on `dev`, looked up once per device with that
+// device current (each GPU holds its own copy at its own address).
+hipError_t scratch_base(int dev, unsigned char** out) {
+    if (dev < 0 || dev >= kScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_base[dev]) {
+        hipError_t e = hipSetDevice(dev);
+        if (e != hipSuccess) return e;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_base[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_base[dev];
+    return hipSuccess;
+}
+
+// The topology of one call, in the shape build_full_ops reads.
+struct TopoView {
+    int n_taxa = 0, n_nodes = 0, root = -1;
+    std::vector<int> left, right;
+};
+
+template <int C>
+void launch_noalloc(hipStream_t st, int nblk, bool tip_lds, const Op* ops, const int* prog, const uint8_t* tips,
+                    float* clv, int* scl, const double* P, const double* roots, double* site, double* partial, int m,
+                    int m_pad, int n_taxa) {
+    if (tip_lds)
+        hipLaunchKernelGGL((prune4_kernel<float, C, true>), dim3(nblk), dim3(64 * C), 0, st, ops, prog, tips, clv, scl,
+                           P, roots, nullptr, site, partial, m, m_pad, n_taxa, nblk, -1, 0, 0, 0, 1, nullptr, nullptr);
+    else
+        hipLaunchKernelGGL((prune4_kernel<float, C, false>), dim3(nblk), dim3(64 * C), 0, st, ops, prog, tips, clv,
+                           scl, P, roots, nullptr, site, partial, m, m_pad, n_taxa, nblk, -1, 0, 0, 0, 1, nullptr,
+                           nullptr);
+}
+
+}  // namespace
+
+extern "C" int phylo_loglik_noalloc(phylo_ctx* c, int n_taxa, int n_sites, int n_states, const uint32_t* masks,
+                                    int n_nodes, int root, const int32_t* left, const int32_t* right,
+                                    const double* eff_len, const double* Q, const double* root_weights,
+                                    const double* pi, int n_cat, const double* cat_rates, const double* cat_weights,
+                                    double* site_lnl, double* total) {
+    if (!c) return PHYLO_ERR_ARG;
+    if (!total || !masks || !eff_len || !Q || !root_weights || !pi || !cat_rates || !cat_weights)
+        return fail(c, PHYLO_ERR_ARG, "loglik_noalloc: null argument");
+    if (n_states < 1 || n_states > 4)
+        return fail(c, PHYLO_ERR_UNSUPPORTED, "loglik_noalloc: 1 to 4 states only (the DNA walker)");
+    if (n_cat != 1 && n_cat != 2 && n_cat != 4 && n_cat != 8)
+        return fail(c, PHYLO_ERR_UNSUPPORTED, "loglik_noalloc: number of rate categories must be 1, 2, 4 or 8");
+    if (n_taxa < 2 || n_sites < 1) return fail(c, PHYLO_ERR_ARG, "loglik_noalloc: need n_taxa >= 2, n_sites >= 1");
+    const char* why = check_topology(n_taxa, n_nodes, root, left, right);
+    if (why) return fail(c, PHYLO_ERR_ARG, std::string("loglik_noalloc: ") + why);
+    for (int v = 0; v < n_nodes; ++v)
+        if (v != root && !std::isfinite(eff_len[v])) return fail(c, PHYLO_ERR_ARG, "loglik_noalloc: non-finite length");
+    for (int i = 0; i < n_states * n_states; ++i)
+        if (!std::isfinite(Q[i])) return fail(c, PHYLO_ERR_ARG, "loglik_noalloc: non-finite Q");
+    for (int k = 0; k < n_cat; ++k)
+        if (!(cat_rates[k] >= 0.0) || !(cat_weights[k] >= 0.0) || !std::isfinite(cat_rates[k]))
+            return fail(c, PHYLO_ERR_ARG, "loglik_noalloc: rates and weights must be finite and >= 0");
+    const uint32_t valid = (1u << n_states) - 1u;
+    for (size_t i = 0; i < (size_t)n_taxa * n_sites; ++i)
+        if ((masks[i] & valid) == 0u) {
+            char buf[160];
+            snprintf(buf, sizeof buf, "taxon %zu lacks a character state for character %zu", i / n_sites,
+                     i % n_sites);
+            return fail(c, PHYLO_ERR_DATA, buf);
+        }
+    if (!phylo_math::is_reversible(n_states, Q, pi))
+        return fail(c, PHYLO_ERR_ARG, "loglik_noalloc: Q is not reversible with respect to pi");
+    std::vector<double> V, lam, Vinv;
+    if (!phylo_math::reversible_eigen(n_states, Q, pi, V, lam, Vinv))
+        return fail(c, PHYLO_ERR_ARG, "loglik_noalloc: eigen-decomposition failed (pi must be positive)");
+
+    // the lean program of phylo_loglik, its stored partials renumbered densely
+    TopoView tv;
+    tv.n_taxa = n_taxa;
+    tv.n_nodes = n_nodes;
+    tv.root = root;
+    tv.left.assign(left, left + n_nodes);
+    tv.right.assign(right, right + n_nodes);
+    std::vector<Op> ops;
+    if (!build_full_ops(&tv, ops)) return fail(c, PHYLO_ERR_ARG, "loglik_noalloc: internal nodes unreachable");
+    drop_unread_stores(ops, 0, ops.size());
+    std::vector<int> dense(n_nodes - n_taxa, -1);
+    int n_store = 0;
+    for (Op& o : ops)
+        if (o.out_slot >= 0) {
+            dense[o.out_slot] = n_store;
+            o.out_slot = n_store++;
+        }
+    for (Op& o : ops) {
+        if (o.l_kind == SRC_CLV) o.l_idx = dense[o.l_idx];
+        if (o.r_kind == SRC_CLV) o.r_idx = dense[o.r_idx];
+        if (o.l_idx < 0 || o.r_idx < 0) return fail(c, PHYLO_ERR_ARG, "loglik_noalloc: program error");
+    }
+
+    // workspace layout, every region 256-byte aligned
+    const int m_pad = (n_sites + kBlock - 1) / kBlock * kBlock, nblk = m_pad / 64, C = n_cat;
+    const size_t n_eig = V.size() + Vinv.size() + lam.size() + (size_t)C + (size_t)n_nodes;
+    const size_t n_roots = kMaxStates + kMaxCat;
+    size_t off = 0;
+    const auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off += (bytes + 255) & ~size_t(255);
+        return at;
+    };
+    const size_t o_tips = take((size_t)n_taxa * m_pad);
+    const size_t o_ops = take(ops.size() * sizeof(Op));
+    const size_t o_prog = take(2 * sizeof(int));
+    const size_t o_eig = take(n_eig * sizeof(double));
+    const size_t o_roots = take(n_roots * sizeof(double));
+    const size_t in_bytes = off;                                    // staged from the host in one copy
+    const size_t o_P = take((size_t)n_nodes * C * 16 * sizeof(double));
+    const size_t o_clv = take((size_t)n_store * C * m_pad * 4 * sizeof(float));
+    const size_t o_scl = take((size_t)n_store * C * m_pad * sizeof(int));
+    const size_t o_part = take((size_t)nblk * sizeof(double));
+    const size_t o_site = take(site_lnl ? (size_t)n_sites * sizeof(double) : 0);
+    const size_t o_total = take(sizeof(double));
+    if (off > kScratchBytes) {
+        char buf[200];
+        snprintf(buf, sizeof buf, "loglik_noalloc: this problem needs %zu bytes of workspace, more than the %zu-byte "
+                 "scratch array", off, kScratchBytes);
+        return fail(c, PHYLO_ERR_ARG, buf);
+    }
+
+    std::vector<unsigned char> in(in_bytes, 0);
+    for (int t = 0; t < n_taxa; ++t) {
+        uint8_t* row = in.data() + o_tips + (size_t)t * m_pad;
+        for (int s = 0; s < n_sites; ++s) row[s] = (uint8_t)(masks[(size_t)t * n_sites + s] & valid);
+        std::fill(row + n_sites, row + m_pad, (uint8_t)0x0F);     // padding sites: all states
+    }
+    std::memcpy(in.data() + o_ops, ops.data(), ops.size() * sizeof(Op));
+    const int prog[2] = {0, (int)ops.size()};
+    std::memcpy(in.data() + o_prog, prog, sizeof prog);
+    double* eig = reinterpret_cast<double*>(in.data() + o_eig);
+    double* w = std::copy(V.begin(), V.end(), eig);
+    w = std::copy(Vinv.begin(), Vinv.end(), w);
+    w = std::copy(lam.begin(), lam.end(), w);
+    w = std::copy(cat_rates, cat_rates + C, w);
+    std::copy(eff_len, eff_len + n_nodes, w);
+    w[root] = 0.0;
+    double* roots = reinterpret_cast<double*>(in.data() + o_roots);
+    for (int i = 0; i < n_states; ++i) roots[i] = root_weights[i];
+    for (int k = 0; k < C; ++k) roots[kRootCatOff + k] = cat_weights[k];
+
+    HIPCHK(c, hipSetDevice(c->device));
+    unsigned char* base = nullptr;
+    HIPCHK(c, scratch_base(c->device, &base));
+    hipStream_t st = c->stream;
+    // nothing in the workspace has to start at zero: every stored partial,
+    // scaler and block sum is written before it is read
+    HIPCHK(c, hipMemcpyAsync(base, in.data(), in_bytes, hipMemcpyHostToDevice, st));
+    const double* dV = reinterpret_cast<const double*>(base + o_eig);
+    const double* dVi = dV + V.size();
+    const double* dl = dVi + Vinv.size();
+    const double* dr = dl + lam.size();
+    const double* d_eff = dr + C;
+    double* dP = reinterpret_cast<double*>(base + o_P);
+    const long nP = (long)n_nodes * C * 16;
+    hipLaunchKernelGGL(pbuild_eigen_kernel, dim3((unsigned)((nP + 255) / 256)), dim3(256), 0, st, dV, dVi, dl, d_eff,
+                       dr, n_states, 4, C, n_nodes, dP);
+    HIPCHK(c, hipGetLastError());
+    const Op* d_ops = reinterpret_cast<const Op*>(base + o_ops);
+    const int* d_prog = reinterpret_cast<const int*>(base + o_prog);
+    const uint8_t* d_tips = base + o_tips;
+    float* d_clv = reinterpret_cast<float*>(base + o_clv);
+    int* d_scl = reinterpret_cast<int*>(base + o_scl);
+    const double* d_roots = reinterpret_cast<const double*>(base + o_roots);
+    double* d_site = site_lnl ? reinterpret_cast<double*>(base + o_site) : nullptr;
+    double* d_part = reinterpret_cast<double*>(base + o_part);
+    double* d_total = reinterpret_cast<double*>(base + o_total);
+    const bool tip_lds = n_taxa <= kTipLdsTaxa;
+    switch (C) {
+    case 1: launch_noalloc<1>(st, nblk, tip_lds, d_ops, d_prog, d_tips, d_clv, d_scl, dP, d_roots, d_site, d_part,
+                              n_sites, m_pad, n_taxa); break;
+    case 2: launch_noalloc<2>(st, nblk, tip_lds, d_ops, d_prog, d_tips, d_clv, d_scl, dP, d_roots, d_site, d_part,
+                              n_sites, m_pad, n_taxa); break;
+    case 4: launch_noalloc<4>(st, nblk, tip_lds, d_ops, d_prog, d_tips, d_clv, d_scl, dP, d_roots, d_site, d_part,
+                              n_sites, m_pad, n_taxa); break;
+    default: launch_noalloc<8>(st, nblk, tip_lds, d_ops, d_prog, d_tips, d_clv, d_scl, dP, d_roots, d_site, d_part,
+                               n_sites, m_pad, n_taxa); break;
+    }
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(final_sum_kernel, dim3(1), dim3(kSumThreads), 0, st, d_part, nblk, d_total);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(total, d_total, sizeof(double), hipMemcpyDeviceToHost, st));
+    if (site_lnl)
+        HIPCHK(c, hipMemcpyAsync(site_lnl, d_site, (size_t)n_sites * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    return PHYLO_OK;
+}
 
 // ---------------------------------------------------------------------------
 // Batched scoring of many independent problems (+ NNI neighbourhoods)
