@@ -18,9 +18,10 @@
 //  * Arithmetic is FP64; storage is FP32 (default) or FP64.  P matrices are
 //    FP64, indexed by the child node and category, and are wave-uniform so
 //    the compiler keeps them in scalar registers.
-//  * Every node's partial is renormalised by an exact power of two (frexp /
-//    ldexp) so its largest entry is in [0.5, 1); the integer exponent is
-//    accumulated into one scaler per (node, category, site).
+//  * Partials are rescaled by exact powers of two (frexp / ldexp): a stored
+//    one so its largest entry is in [0.5, 1), a carried FP64 one (DNA
+//    walker) only when its largest entry falls below 2^-256; the integer
+//    exponents add up in one scaler per (node, category, site).
 //  * Per-site lnL = log(sum_c w_c sum_i rw_i L_ci 2^scaler_c) in FP64,
 //    reduced per workgroup with wavefront shuffles, then summed in a fixed
 //    order with compensation by a second one-wave launch.
@@ -245,6 +246,16 @@ __device__ inline int max_exponent(const double (&L)[4]) {
     return e;
 }
 
+// The largest high word of four doubles, as signed integers: for positive
+// values it orders like the values (negative ones never win).
+__device__ inline int max_hi_word(const double (&L)[4]) {
+    return max(max(__double2hiint(L[0]), __double2hiint(L[1])), max(__double2hiint(L[2]), __double2hiint(L[3])));
+}
+
+// High word of 2^-256: carried DNA partials whose largest entry is below it
+// are rescaled (walk4).
+constexpr int kLazyRescaleHi = (1023 - 256) << 20;
+
 // One child's operand for a walker op, by source kind (compile-time).
 template <int K, typename T, int C, bool TL>
 __device__ inline void child4(int idx, const uint8_t* __restrict__ tips, const uint8_t* tl, const T* plane,
@@ -306,10 +317,12 @@ __device__ inline void children4(const Op& op, const uint8_t* __restrict__ tips,
 // A workgroup owns 64 consecutive sites; wave c of the workgroup computes rate
 // category c for them, so the P matrices a wave needs are uniform across it
 // (scalar loads) and each lane holds one float4 partial.  Each (node, site,
-// category) partial carries its own power-of-two scaler (the frexp exponent
-// of its maximum), so the waves of a workgroup walk the program without
-// waiting for each other; they meet only at the root, where the categories'
-// sums are brought to a common exponent.
+// category) partial carries its own power-of-two scaler, so the waves of a
+// workgroup walk the program without waiting for each other; they meet only
+// at the root, where the categories' sums are brought to a common exponent.
+// A stored partial is scaled so its largest entry is in [0.5, 1) (the
+// scaler is the frexp exponent of its maximum); a carried one is rescaled
+// only when it drops below 2^-256.
 // walk4 runs ops[k0, k1) for site s and category cat; at the root op it sets
 // root_lc = w_cat sum_i rw_i L_i and root_sc, the category's scaler, so that
 // its share of the site likelihood is root_lc 2^root_sc.
@@ -354,20 +367,32 @@ __device__ inline void walk4(const Op* __restrict__ ops, int k0, int k1, const u
             default: PHYLO_SHAPE(3, 0) PHYLO_SHAPE(3, 1) PHYLO_SHAPE(3, 2) PHYLO_SHAPE(3, 3) PHYLO_SHAPE(3, 4)
 #undef PHYLO_SHAPE
         }
-        // this category's power-of-two scaler: the partial's maximum to [0.5, 1)
-        const int e = max_exponent(L);
-        sc += e;
+        // Carried results are FP64 and rescaled lazily: only when the
+        // largest entry falls below 2^-256 (rare; a lane-divergent branch
+        // costs the others one compare), so an op's children are each at
+        // least 2^-256 and their product stays far above the FP64 normal
+        // range.  (Rescaling by a power of two is exact, so where it
+        // happens changes no value.)
+        if (max_hi_word(L) < kLazyRescaleHi) {
+            const int e = max_exponent(L);
+            sc += e;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) L[i] = ldexp(L[i], -e);
+            for (int i = 0; i < 4; ++i) L[i] = ldexp(L[i], -e);
+        }
         if (op.out_slot >= 0) {
-            // most results are only carried (the program stores what it reads
-            // back), so the rounding sits here, on the few stored ones
+            // a stored partial is brought to [0.5, 1) with its own scaler
+            // (the carried value keeps its scale, so programs that store
+            // and programs that carry a node compute the same values), then
+            // rounded to T: most results are only carried (the program
+            // stores what it reads back), so the rounding sits here, on the
+            // few stored ones
+            const int e = max_exponent(L);
             const uint32_t key = round_key(s, op.l_node, op.r_node, cat);
             T r[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) r[i] = (T)round_cut(L[i], key + 0x9e3779b9u * i, T());
+            for (int i = 0; i < 4; ++i) r[i] = (T)round_cut(ldexp(L[i], -e), key + 0x9e3779b9u * i, T());
             store4(const_cast<T*>(plane) + op.out_slot * slot_stride, r);
-            scl[((size_t)op.out_slot * C + cat) * m_pad + s] = sc;
+            scl[((size_t)op.out_slot * C + cat) * m_pad + s] = sc + e;
         }
         // carry A takes every result (after an op to carry B nothing reads
         // the old A), so no select between the carries
@@ -380,11 +405,16 @@ __device__ inline void walk4(const Op* __restrict__ ops, int k0, int k1, const u
             sB = sc;
         }
         if (op.flags & OP_ROOT) {
+            // the root partial is brought to [0.5, 1) first, so the site
+            // value does not depend on where the walk rescaled (a program
+            // that reads a stored child and one that carries it reach the
+            // root with the same mantissas)
+            const int e = max_exponent(L);
             double lc = 0.0;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) lc = fma(roots[i], L[i], lc);
+            for (int i = 0; i < 4; ++i) lc = fma(roots[i], ldexp(L[i], -e), lc);
             root_lc = lc * roots[kRootCatOff + cat];
-            root_sc = sc;
+            root_sc = sc + e;
         }
     }
 }
@@ -1496,6 +1526,21 @@ int prepare(phylo_ctx* c) {
     return PHYLO_OK;
 }
 
+// Workgroups per CU below which a whole-tree launch has too few waves to
+// hide its dependent chain: every site block is then walked as subtree
+// pieces plus the top program (the tail split applied to all blocks), which
+// multiplies the independent waves by the number of pieces.
+constexpr int kSplitAllBelowPerCu = 4;
+
+// Site blocks of a whole-tree launch walked as whole-tree workgroups; the
+// rest (the tail) go through the split programs.  Full rounds over the CUs
+// stay whole unless there are too few of them.
+int split_main_blocks(const phylo_ctx* c, int nblk) {
+    if (c->n_cu <= 0) return nblk;
+    if (nblk < kSplitAllBelowPerCu * c->n_cu) return 0;
+    return nblk / c->n_cu * c->n_cu;
+}
+
 template <typename T, int C>
 void launch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
     const auto* tips = reinterpret_cast<const uint8_t*>(c->d_tips);
@@ -1505,8 +1550,8 @@ void launch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
     // path-to-root programs of an NNI neighbourhood
     if (c->run_whole_tree && c->n_taxa <= kTipLdsTaxa) {
         const int nblk = (int)grid.x;
-        const int main_b = c->n_cu > 0 ? nblk / c->n_cu * c->n_cu : 0, tail = nblk - main_b;
-        if (c->tail_split && c->split_K > 0 && grid.y == 1 && main_b > 0 && tail > 0) {
+        const int main_b = split_main_blocks(c, nblk), tail = nblk - main_b;
+        if (c->tail_split && c->split_K > 0 && grid.y == 1 && tail > 0) {
             // the main blocks and the tail's pieces, then the tail's top program
             const int base = c->split_base[c->run_full ? 0 : 1], K = c->split_K;
             hipLaunchKernelGGL((prune4_kernel<T, C, true>), dim3(main_b + K * tail), dim3(64 * C), 0, c->stream,
@@ -1555,8 +1600,8 @@ void launch_mfma(phylo_ctx* c, dim3 grid, double* site_lnl) {
     const uint32_t full = c->S == 32 ? 0xFFFFFFFFu : ((1u << c->S) - 1u);
     if (c->run_whole_tree && c->n_taxa <= kMfmaTipLdsTaxa) {
         const int nblk = (int)grid.x;
-        const int main_b = c->n_cu > 0 ? nblk / c->n_cu * c->n_cu : 0, tail = nblk - main_b;
-        if (c->tail_split && c->split_K > 0 && grid.y == 1 && main_b > 0 && tail > 0) {
+        const int main_b = split_main_blocks(c, nblk), tail = nblk - main_b;
+        if (c->tail_split && c->split_K > 0 && grid.y == 1 && tail > 0) {
             // the main blocks and the tail's pieces, then the tail's top program
             const int base = c->split_base[c->run_full ? 0 : 1], K = c->split_K;
             hipLaunchKernelGGL((prune_mfma_kernel<T, C, V, kMfmaGroups, true>), dim3(main_b + K * tail), dim3(64 * C),
