@@ -44,6 +44,26 @@ def alg_bytes_per_update(n, S=4, C=4):
 
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
+FP64_VALU_PEAK_TFLOPS = 78.6     # MI355X FP64 vector peak (equal to its dense FP64 matrix peak)
+
+
+def flops_per_update(S):
+    """FP64 flops of one (site, node, category) update: two P x L products
+    (2 S^2 multiply-adds each) and the elementwise product of the children."""
+    return 2 * 2 * S * S + S
+
+
+def host_cores():
+    """CPUs this process may run on (its affinity mask), capped at the
+    CPU share the environment grants a job (OMP_NUM_THREADS when set): the
+    count of concurrent CPU shards the baseline runs."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    cap = int(share) if share.isdigit() and int(share) > 0 else avail
+    return max(1, min(avail, cap)), avail
 
 
 def cpu_baseline(wl, budget_s=20.0):
@@ -57,7 +77,7 @@ def cpu_baseline(wl, budget_s=20.0):
     n = t.n_tips
     harness = O.REF_HARNESS
     m_sample = 1200
-    cores = max(1, min(16, os.cpu_count() or 1))
+    cores, avail = host_cores()
     if os.path.exists(harness) and os.access(harness, os.X_OK):
         with tempfile.TemporaryDirectory() as d:
             open(os.path.join(d, "t.nwk"), "w").write(t.newick() + "\n")
@@ -87,10 +107,13 @@ def cpu_baseline(wl, budget_s=20.0):
             rate_all = sum(updates / s for s in many)
             return {"value": rate_all, "unit": "updates/s", "cores": cores, "kind": "reference",
                     "single_core": rate_one,
+                    "cores_available": avail,
                     "sample": f"phylommand simpleML::calculate_log_likelihood compiled -O2 from the reference "
                               f"sources, {n} taxa x {m_sample} sites per core, {cores} concurrent site shards "
-                              f"(1 core: {rate_one:.3g} updates/s); 1 rate category, the reference has no gamma "
-                              f"and its per-category cost is this; best of 3 calls per shard; {wall:.1f}s wall"}
+                              f"({avail} CPUs in the affinity mask, OMP_NUM_THREADS="
+                              f"{os.environ.get('OMP_NUM_THREADS', 'unset')}; 1 core: {rate_one:.3g} updates/s); "
+                              f"1 rate category, the reference has no gamma and its per-category cost is this; "
+                              f"best of 3 calls per shard; {wall:.1f}s wall"}
     masks = wl.masks[:, :m_sample]
     # restatement of the reference's cost model
     mb = np.zeros((t.n_nodes, m_sample), np.uint32)
@@ -182,11 +205,12 @@ def profiled_traffic(name):
         return None
 
 
-def bench_protein(args, P, synth):
-    """20-state protein, 64-taxon Yule tree, 20,000 sites, 190 exchangeabilities
-    ~ Gamma(1,1), frequencies ~ Dirichlet(2x20), gamma shape 1.0 (4 categories),
-    3% gaps."""
-    m = args.sites or 20_000
+def measure_protein(P, synth, steps, warmup, m=20_000):
+    """The 20-state walker on its workload: 64-taxon Yule tree, m sites, 190
+    exchangeabilities ~ Gamma(1,1), frequencies ~ Dirichlet(2x20), gamma shape
+    1.0 (4 categories), 3% gaps.  Returns the lnL, the wall time per step and
+    the mean kernel time (HIP events on the library's stream) with the roofline
+    on the FP64 matrix cores."""
     wl = synth.gtr_gamma_workload(n_tips=64, m=m, S=20, C=4, alpha=1.0, gap_frac=0.03, seed=4,
                                   rate_dist="gamma", freq_conc=2.0, mean_blen=0.05)
     t = wl.tree
@@ -196,37 +220,53 @@ def bench_protein(args, P, synth):
     ctx.set_edge_lengths(t.blen)
     ctx.set_model(wl.Q, wl.pi, wl.pi, P.P_EIGEN)
     ctx.set_categories(wl.cat_rates, wl.cat_weights)
-    for _ in range(args.warmup + 1):
+    for _ in range(warmup + 1):
         ctx.loglik()
     ctx.synchronize()
     kms = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         lnl = ctx.loglik()
         kms.append(ctx.last_kernel_ms())
     ctx.synchronize()
     el = time.perf_counter() - t0
+    ctx.close()
     n, C, S = 64, 4, 20
     updates = (n - 1) * m * C
     ks = float(np.mean(kms)) * 1e-3
+    # Matrix-core work: a stored or carried child's P x L is an FP64 MFMA chain
+    # (2 S^2 flops per site and category); a tip child's is a gather of P
+    # columns (no MFMA).  Counted over the sites, not the padding.
+    internal_children = sum(int(c >= n) for v in range(t.n_nodes) for c in (t.left[v], t.right[v]) if c >= 0)
+    mfma_flops = internal_children * m * C * 2 * S * S
     per_site = C * S * 4 * (n - 2) + n + C * S * 4 * (n - 1) + 4 * (n - 1) + 4 * (n - 2)
     bpu = per_site / ((n - 1) * C)
-    flops = 2 * 2 * S * S + S        # useful FP64 flops per update (no padding counted)
-    return {"metric": "CLV updates/s (site*node*cat)", "value": updates * args.steps / el, "unit": "updates/s",
-            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
+    traffic = profiled_traffic("prune_mfma_protein20_g4_64x20k") if m == 20_000 else None
+    tf = mfma_flops / ks / 1e12
+    return {"lnL": lnl, "updates": updates, "ms_per_step": el / steps * 1e3, "kernel_ms": ks * 1e3,
+            "value": updates * steps / el,
+            "roofline": {"bound": "mfma", "achieved": tf, "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": tf / FP64_MFMA_PEAK_TFLOPS, "traffic": traffic,
+                         "kernel": "prune_mfma_kernel<float,4,5,2,true>", "kernel_ms": ks * 1e3,
+                         "mfma_flops_per_step": mfma_flops,
+                         "useful_flops_per_update": flops_per_update(S), "alg_bytes_per_update": bpu,
+                         "hbm_gbs": traffic / ks / 1e9 if traffic else None,
+                         "hbm_frac": traffic / ks / 1e9 / HBM_PEAK_GBS if traffic else None}}
+
+
+def bench_protein(args, P, synth):
+    """20-state protein, 64-taxon Yule tree, 20,000 sites, 190 exchangeabilities
+    ~ Gamma(1,1), frequencies ~ Dirichlet(2x20), gamma shape 1.0 (4 categories),
+    3% gaps."""
+    m = args.sites or 20_000
+    r = measure_protein(P, synth, args.steps, args.warmup, m)
+    return {"metric": "CLV updates/s (site*node*cat)", "value": r["value"], "unit": "updates/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic (seeded Yule tree, 20-state simulation)",
-            "config": {"workload": f"20-state protein + G4, 64 taxa Yule, {m} sites, 3% gaps", "n_taxa": n,
-                       "n_sites": m, "n_states": S, "n_cat": C, "clv_storage": "f32", "lnL": lnl},
-            # CLV x P runs on the FP64 matrix cores: 10 flop/B against a ridge of
-            # 78.6 TF / 8 TB/s = 9.8 flop/B, and with 20 states padded to 32 rows
-            # the MFMA roof (164 us) sits above the HBM one (101 us)
-            "roofline": {"bound": "mfma", "achieved": flops * updates / ks / 1e12, "peak": FP64_MFMA_PEAK_TFLOPS,
-                         "unit": "TFLOP/s", "frac": flops * updates / ks / 1e12 / FP64_MFMA_PEAK_TFLOPS,
-                         "traffic": profiled_traffic("prune_mfma_protein20_g4_64x20k") if m == 20_000 else None,
-                         "kernel": "prune_mfma_kernel<float,4,5,2,true>", "kernel_ms": ks * 1e3,
-                         "alg_flops_per_update": flops, "alg_bytes_per_update": bpu,
-                         "hbm_gbs": bpu * updates / ks / 1e9, "hbm_frac": bpu * updates / ks / 1e9 / HBM_PEAK_GBS}}
+            "config": {"workload": f"20-state protein + G4, 64 taxa Yule, {m} sites, 3% gaps", "n_taxa": 64,
+                       "n_sites": m, "n_states": 20, "n_cat": 4, "clv_storage": "f32", "lnL": r["lnL"]},
+            "roofline": r["roofline"]}
 
 
 def bench_nni(args, P, synth):
@@ -267,18 +307,13 @@ def bench_batch(args, P, synth, n_inst=512):
     [500, 50000], per-instance Dirichlet parameters, each with its full NNI
     neighbourhood, scored by the persistent batch launches replayed from one
     captured HIP graph."""
-    rng = np.random.default_rng(3)
-    b = P.Batch(4, P.PREC_FP32)
     t0 = time.time()
-    for i in range(n_inst):
-        n = int(rng.integers(8, 257))
-        m = int(np.exp(rng.uniform(np.log(500), np.log(50_000))))
-        wl = synth.gtr_gamma_workload(n_tips=n, m=m, C=4, alpha=float(rng.uniform(0.3, 2.0)), gap_frac=0.05,
-                                      seed=10_000 + i, mean_blen=0.05)
+    wls = synth.batch_workloads(n_inst, workers=host_cores()[0])
+    print(f"[bench] batch: {n_inst} instances simulated ({time.time() - t0:.0f}s)", file=sys.stderr, flush=True)
+    b = P.Batch(4, P.PREC_FP32)
+    for wl in wls:
         t = wl.tree
         b.add(wl.masks, t.left, t.right, t.root, t.blen, wl.Q, wl.pi, wl.cat_rates, wl.cat_weights, nni_mode=1)
-        if i % 64 == 63:
-            print(f"[bench] batch: {i + 1} instances added ({time.time() - t0:.0f}s)", file=sys.stderr, flush=True)
     b.finalize(autotune=True)
     for _ in range(args.warmup):
         b.run()
@@ -355,6 +390,7 @@ def main():
     ap.add_argument("--taxa", type=int, default=128)
     ap.add_argument("--fp64", action="store_true", help="store partials in FP64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-protein", action="store_true", help="dna: skip the 20-state walker's extra keys")
     ap.add_argument("--workload", default="dna", choices=["dna", "protein", "nni", "batch", "learn"])
     ap.add_argument("--train", type=int, default=200_000, help="learn: training alignments")
     ap.add_argument("--heldout", type=int, default=1000, help="learn: held-out fits")
@@ -446,7 +482,7 @@ def main():
     value = updates * args.steps / elapsed
     kernel_s = float(np.mean(kms)) * 1e-3
     bpu = alg_bytes_per_update(n, 4, C) * (2 if args.fp64 else 1)
-    achieved = bpu * updates / kernel_s / 1e9
+    achieved = flops_per_update(4) * updates / kernel_s / 1e12
     traffic = profiled_traffic("prune4_gtr_g4_128x100k") if (not args.fp64 and m == 100_000 and n == 128) else None
     out = {
         "metric": "CLV updates/s (site*node*cat)",
@@ -467,33 +503,43 @@ def main():
         "end_to_end": {"ms_per_step": e2e_ms, "steps": e2e_steps,
                        "what": "set_edge_lengths with new lengths (host-to-device copy of the eigen system "
                                "and edge lengths, P rebuilt on the device) + loglik, alternating two length sets"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+        # What binds the DNA walker is FP64 arithmetic issue, not HBM: it keeps
+        # all but a few partials in registers and LDS, so it moves a tenth of
+        # the bytes a store-every-partial design would (alg_bytes_per_update),
+        # and the counters show HBM far from busy (hbm_frac).  achieved is the
+        # update's FP64 flops (2 P x L products + the elementwise product) per
+        # second of kernel time (both launches of a tail split, gap included).
+        "roofline": {"bound": "valu_fp64", "achieved": achieved, "peak": FP64_VALU_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_VALU_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "prune4_kernel<float,4,true>", "kernel_ms": kernel_s * 1e3,
-                     # kernel_ms spans both launches of an evaluation when the site
-                     # blocks leave a tail over the CUs (main blocks + subtree pieces,
-                     # then the tail's top program), gap included
+                     "flops_per_update": flops_per_update(4),
+                     "hbm_gbs": traffic / kernel_s / 1e9 if traffic else None,
+                     "hbm_frac": traffic / kernel_s / 1e9 / HBM_PEAK_GBS if traffic else None,
                      "alg_bytes_per_update": bpu,
-                     # the algorithmic bytes assume a CLV store per node: every internal
-                     # partial written once and every stored child read back.  The kernel
-                     # keeps most partials in registers / LDS, so it moves far fewer bytes
-                     # (traffic) and frac can pass 1; traffic_gbs is what HBM really serves.
-                     "traffic_gbs": traffic / kernel_s / 1e9 if traffic else None},
+                     "alg_bytes_gbs": bpu * updates / kernel_s / 1e9},
     }
     try:
-        cgbs = copy_bandwidth_subprocess()
-        out["roofline"]["copy_gbs"] = cgbs
-        out["roofline"]["frac_of_copy"] = achieved / cgbs
+        out["roofline"]["copy_gbs"] = copy_bandwidth_subprocess()
     except Exception as e:  # optional calibration
         out["roofline"]["copy_gbs"] = None
         print(f"[bench] copy bandwidth not measured: {e}", file=sys.stderr)
+    if not args.no_protein:
+        # the 20-state walker on its own workload, timed the same way: context
+        # for the DNA line, not part of its value
+        try:
+            r = measure_protein(P, synth, min(args.steps, 200), min(args.warmup, 300))
+            out["protein_20_state"] = {"workload": "20-state protein + G4, 64 taxa Yule, 20000 sites, 3% gaps",
+                                       "updates_per_s": r["value"], "ms_per_step": r["ms_per_step"],
+                                       "kernel_ms": r["kernel_ms"], "lnL": r["lnL"], "roofline": r["roofline"]}
+        except Exception as e:
+            out["protein_20_state"] = {"error": str(e)}
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(wl)
         except Exception as e:  # the GPU figure stands on its own
             out["cpu_baseline"] = {"value": None, "error": str(e)}
         try:
-            out["cpu_baseline"]["fair_cpu"] = fair_cpu(wl, out["cpu_baseline"].get("cores") or 16)
+            out["cpu_baseline"]["fair_cpu"] = fair_cpu(wl, out["cpu_baseline"].get("cores") or host_cores()[0])
         except Exception as e:
             out["cpu_baseline"]["fair_cpu"] = {"value": None, "error": str(e)}
     print(json.dumps(out))

@@ -242,6 +242,22 @@ gtr_fit fit_gtr_gamma(phylo_ctx* ctx, int n_nodes, const double* rel_len, int n_
     // the optimum in ten dimensions.
     nm_result r = nelder_mead(f, x0, lo, hi, opt);
     int evals = r.evals;
+    // A simplex that collapsed onto a bound (one coordinate pinned at lo or
+    // hi) can stall far from the optimum, and a small restart simplex stays
+    // stuck with it: restart once from the best point with a simplex of the
+    // first run's size, and keep it if it gains.
+    const auto at_bound = [&](const std::vector<double>& x) {
+        for (size_t i = 0; i < x.size(); ++i)
+            if (x[i] <= lo[i] + 1e-9 || x[i] >= hi[i] - 1e-9) return true;
+        return false;
+    };
+    if (at_bound(r.x) && evals < opt.max_evals) {
+        nm_options o = opt;
+        o.max_evals = opt.max_evals - evals;
+        const nm_result r2 = nelder_mead(f, r.x, lo, hi, o);
+        evals += r2.evals;
+        if (r2.f > r.f) r = r2;
+    }
     for (int restart = 0; restart < opt.max_restarts && evals < opt.max_evals; ++restart) {
         nm_options o = opt;
         o.max_evals = opt.max_evals - evals;

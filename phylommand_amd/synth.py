@@ -267,3 +267,42 @@ def gtr_gamma_workload(n_tips=128, m=100_000, mean_blen=0.05, alpha=0.5, C=4, ga
     states = simulate(t, Q, pi, m, rng, cr)
     masks = masks_from_states(states[: t.n_tips], gap_frac, rng, S)
     return Workload(t, masks, Q, pi, rates, cr, np.full(C, 1.0 / C), alpha)
+
+
+# ---- the batched-scoring workload: many alignments of uneven shape ---------
+
+def batch_instance_params(n_inst: int = 512, seed: int = 3):
+    """Per-instance (taxa, sites, gamma shape) of the batched workload: taxa ~
+    U{8..256}, sites log-uniform in [500, 50000], shape ~ U(0.3, 2); instance
+    i is gtr_gamma_workload(seed=10000 + i) with these (mean length 0.05,
+    4 categories, 5% gaps)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_inst):
+        n = int(rng.integers(8, 257))
+        m = int(np.exp(rng.uniform(np.log(500), np.log(50_000))))
+        out.append((n, m, float(rng.uniform(0.3, 2.0))))
+    return out
+
+
+def batch_instance(i: int, n: int, m: int, alpha: float) -> Workload:
+    return gtr_gamma_workload(n_tips=n, m=m, C=4, alpha=alpha, gap_frac=0.05, seed=10_000 + i, mean_blen=0.05)
+
+
+def _batch_instance_packed(args):
+    i, n, m, alpha = args
+    wl = batch_instance(i, n, m, alpha)
+    wl.masks = wl.masks.astype(np.uint8)          # DNA state sets fit a byte: a quarter of the transfer
+    return wl
+
+
+def batch_workloads(n_inst: int = 512, seed: int = 3, workers: int = 16):
+    """The batched workload's instances, simulated by `workers` spawned
+    processes (masks as uint8).  Spawned rather than forked, so a caller that
+    has already initialised the GPU can use it."""
+    import multiprocessing as mp
+    params = [(i, *p) for i, p in enumerate(batch_instance_params(n_inst, seed))]
+    if workers <= 1:
+        return [_batch_instance_packed(a) for a in params]
+    with mp.get_context("spawn").Pool(workers) as pool:
+        return pool.map(_batch_instance_packed, params, chunksize=4)
