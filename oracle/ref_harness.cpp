@@ -16,6 +16,9 @@
 //   tmbl     TIMEARG B1 D1 [B2 D2 ...]              get_time_mod_branch_length dumps
 //   ratearg  clade|time ARG                        the rate_model pars_clade_rates_arg /
 //            pars_rates_and_times_arg build, through its getters
+//   nexus    TREE                                   print_nexus_tree_intro / print_tree_to_nexus /
+//            print_nexus_tree_end, empty then filled Translate table
+//   parsex   TREE                                   print_newick of trees read with "[&...]" kept
 //   nodes    TREE FASTA ALPHA TR SPEC PARAMS FREQS NSITES
 //            draw_normalized_likelihood_on_nodes + print_newick
 //   parse    TREE                                   print_newick of each parsed tree
@@ -200,6 +203,27 @@ int main(int argc, char** argv) {
             model.set_P_matrix(atof(argv[a]));
             for (unsigned int i = 0; i < S; ++i)
                 for (unsigned int j = 0; j < S; ++j) printf("%.17g%c", model.get_P_value(i, j), j + 1 < S ? ' ' : '\n');
+        }
+        return 0;
+    }
+    if (mode == "nexus" || mode == "parsex") {
+        // NEXUS writer (an empty Translate table, filled by the intro, then
+        // the filled one), or newick of trees read with "[&...]" annotations
+        std::stringstream ts(slurp(argv[2]));
+        std::map<std::string, std::string> no_translation;
+        while (true) {
+            tree t;
+            t.tree_file_parser(ts, no_translation, mode == "parsex");
+            if (t.empty()) break;
+            if (mode == "parsex") { t.print_newick(); continue; }
+            std::map<std::string, std::string> tt;
+            std::string name = "tree1";
+            t.print_nexus_tree_intro(tt);
+            t.print_tree_to_nexus(name, true, true, tt);
+            t.print_nexus_tree_end();
+            t.print_nexus_tree_intro(tt);
+            t.print_tree_to_nexus(name, false, true, tt);
+            t.print_nexus_tree_end();
         }
         return 0;
     }

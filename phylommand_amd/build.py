@@ -58,9 +58,25 @@ def build_host():
     os.replace(tmp, HOST_DRIVER)
 
 
+TREEATOR_DIR = os.path.join(HOST, "_treeator")
+REF_SRC = os.environ.get("PHYLOMMAND_SRC", "/root/reference/src")
+
+
+def build_treeator():
+    """phylommand's own treeator (src/treeator.cpp, unmodified) built against
+    the host classes by host/treeator.mk, when the phylommand sources are
+    present; the binary lands in host/_treeator/."""
+    if not os.path.isfile(os.path.join(REF_SRC, "treeator.cpp")):
+        print(f"[build] {REF_SRC}/treeator.cpp not found: treeator binding not built", flush=True)
+        return None
+    run(["make", "-s", "-f", os.path.join(HOST, "treeator.mk"), "REF=" + REF_SRC, "OUT=" + TREEATOR_DIR])
+    return os.path.join(TREEATOR_DIR, "treeator")
+
+
 def build_all():
     build_hip()
     build_host()
+    build_treeator()
 
 
 if __name__ == "__main__":

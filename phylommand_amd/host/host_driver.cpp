@@ -8,6 +8,8 @@
 //   nni      TREE          the tree and its treebender --nni all variants
 //   nniscore TREE FASTA ALPHA TR SPEC PARAMS FREQS MODE     NNI neighbourhood lnL
 //   parse    TREE          print_newick of every tree in the file
+//   parsex   TREE          the same with "[&...]" annotations kept in the labels
+//   nexus    TREE          NEXUS writer: intro, tree, end (empty, then filled Translate table)
 //   tmbl     TIMEARG B D [B D ...]
 //   ratearg  clade|time ARG   the rate model a -V / time-rate argument builds
 //   pmatrix  S TR SPEC PARAMS FREQS T [T ...]
@@ -171,6 +173,27 @@ int main(int argc, char** argv) {
         }
         return 0;
     }
+    if (mode == "nexus" || mode == "parsex") {
+        // NEXUS writer (an empty Translate table, filled by the intro, then
+        // the filled one), or newick of trees read with "[&...]" annotations
+        std::stringstream ts(slurp(argv[2]));
+        std::map<std::string, std::string> no_translation;
+        while (true) {
+            tree t;
+            t.tree_file_parser(ts, no_translation, mode == "parsex");
+            if (t.empty()) break;
+            if (mode == "parsex") { t.print_newick(); continue; }
+            std::map<std::string, std::string> tt;
+            std::string name = "tree1";
+            t.print_nexus_tree_intro(tt);
+            t.print_tree_to_nexus(name, true, true, tt);
+            t.print_nexus_tree_end();
+            t.print_nexus_tree_intro(tt);
+            t.print_tree_to_nexus(name, false, true, tt);
+            t.print_nexus_tree_end();
+        }
+        return 0;
+    }
     if (mode == "ratearg") {
         // the model a -V (clade) or time-rate argument builds, through the getters
         rate_model rate_mod;
@@ -223,7 +246,8 @@ int main(int argc, char** argv) {
         t.tree_file_parser(ts);
         t.set_simulation_seed((uint64_t)atoll(argv[9]));
         std::vector<double> freq = model.frequencies();
-        for (auto& r : t.simulate_chars(freq, (unsigned int)atoi(argv[8]), model)) {
+        const std::vector<character_vector> rows = t.simulate_chars(freq, (unsigned int)atoi(argv[8]), model);
+        for (const auto& r : rows) {
             std::cout << '>' << r.get_taxon() << '\n';
             for (unsigned int j = 0; j < r.n_char(); ++j) std::cout << alphabet::translate_bitset(r.character(j), d.alpha);
             std::cout << '\n';

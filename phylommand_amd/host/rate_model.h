@@ -56,4 +56,10 @@ class rate_model {
 
 }  // namespace phylo_amd
 
+#ifdef PHYLO_AMD_AS_REFERENCE
+// Building phylommand's own sources against these classes (the treeator
+// binding): the unqualified name they use.
+using phylo_amd::rate_model;
+#endif
+
 #endif

@@ -2,6 +2,7 @@
 #include "simpleML.h"
 
 #include <cmath>
+#include <cstdlib>
 #include <iostream>
 #include <random>
 #include <sstream>
@@ -9,6 +10,18 @@
 #include "phylo_hip.h"
 
 namespace phylo_amd {
+
+simpleML::simpleML() {
+    const char* b = std::getenv("PHYLO_AMD_P_BUILDER");
+    if (b) {
+        const std::string v(b);
+        if (v == "refcompat") builder_ = P_REFCOMPAT;
+        else if (v == "eigen") builder_ = P_EIGEN;
+        else if (v == "pade") builder_ = P_PADE;
+    }
+    const char* p = std::getenv("PHYLO_AMD_PRECISION");
+    if (p) fp64_ = std::string(p) == "fp64";
+}
 
 simpleML::~simpleML() {
     if (ctx_) phylo_destroy(ctx_);
@@ -230,9 +243,9 @@ void simpleML::draw_normalized_likelihood_on_nodes() {
 // then on every edge a state drawn from row P[ancestor] of P(raw branch
 // length), on the GPU (phylo_simulate).  Rows are returned for every labelled
 // non-root node in preorder, as the reference does.
-std::vector<character_vector> simpleML::simulate_chars(const std::vector<double>& charfreq, const unsigned int n_char,
-                                                       sub_model& model) {
-    std::vector<character_vector> out;
+simulated_rows simpleML::simulate_chars(const std::vector<double>& charfreq, const unsigned int n_char,
+                                        sub_model& model) {
+    simulated_rows out;
     if (!ctx_ && phylo_create(0, &ctx_)) {
         err_ = "no GPU available for libphylo_hip";
         std::cerr << err_ << std::endl;
@@ -289,15 +302,11 @@ std::vector<character_vector> simpleML::simulate_chars(const std::vector<double>
     }
     for (int v : pre) {
         if (v == root_ || !nodes_[v].has_label || nodes_[v].label.empty()) continue;
-        character_vector row;
+        std::vector<state_set> row(n_char);
         const uint8_t* st = states.data() + (size_t)lib[v] * n_char;
-        for (unsigned int c = 0; c < n_char; ++c) {
-            state_set x;
-            x.set(st[c]);
-            row.add_character(x);
-        }
-        row.set_taxon(nodes_[v].label);
-        out.push_back(row);
+        for (unsigned int c = 0; c < n_char; ++c) row[c].set(st[c]);
+        out.taxa.push_back(nodes_[v].label);
+        out.rows.push_back(std::move(row));
     }
     return out;
 }

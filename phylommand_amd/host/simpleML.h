@@ -23,15 +23,40 @@
 #include "io.h"
 #include "rate_model.h"
 #include "sub_model.h"
-#include "tree.h"
+#include "phylo_tree.h"
 
 struct phylo_ctx;
 
 namespace phylo_amd {
 
+// Simulated characters: per taxon its name and one state set per character.
+// Converts to a std::vector of any character-vector type with set_taxon and
+// add_character -- this library's character_vector, or the caller's own
+// (treeator assigns the result to its vector<character_vector>,
+// treeator.cpp:735).
+struct simulated_rows {
+    std::vector<std::string> taxa;
+    std::vector<std::vector<state_set>> rows;
+    template <class CV>
+    operator std::vector<CV>() const {
+        std::vector<CV> out(taxa.size());
+        for (size_t i = 0; i < taxa.size(); ++i) {
+            out[i].set_taxon(taxa[i]);
+            for (const state_set& x : rows[i]) out[i].add_character(x);
+        }
+        return out;
+    }
+};
+
 class simpleML : public tree {
   public:
-    simpleML() = default;
+    // The engine options start from the environment, so a program built
+    // against these classes unchanged (treeator) can choose them:
+    // PHYLO_AMD_P_BUILDER = refcompat | eigen | pade (default: eigen for
+    // reversible models, pade otherwise) and PHYLO_AMD_PRECISION = fp64 |
+    // fp32 (default fp32).  refcompat with fp64 reproduces the reference's
+    // own values to about 1e-11.
+    simpleML();
     ~simpleML() override;
     simpleML(const simpleML&) = delete;
     simpleML& operator=(const simpleML&) = delete;
@@ -47,8 +72,7 @@ class simpleML : public tree {
     double calculate_likelihood_rate_change_in_time(rate_model& rate_mod, sub_model& model) {
         return calculate_log_likelihood(model, rate_mod);
     }
-    std::vector<character_vector> simulate_chars(const std::vector<double>& charfreq, const unsigned int n_char,
-                                                 sub_model& model);
+    simulated_rows simulate_chars(const std::vector<double>& charfreq, const unsigned int n_char, sub_model& model);
     // Appends "[&P={...},P={...}]" (per site, L/sum with the running sum of the
     // reference, simpleML.cpp:199-231) to every internal node's label.
     void draw_normalized_likelihood_on_nodes();
@@ -88,5 +112,11 @@ class simpleML : public tree {
 };
 
 }  // namespace phylo_amd
+
+#ifdef PHYLO_AMD_AS_REFERENCE
+// Building phylommand's own sources against these classes (the treeator
+// binding): the unqualified name they use.
+using phylo_amd::simpleML;
+#endif
 
 #endif

@@ -78,4 +78,10 @@ class sub_model {
 
 }  // namespace phylo_amd
 
+#ifdef PHYLO_AMD_AS_REFERENCE
+// Building phylommand's own sources against these classes (the treeator
+// binding): the unqualified name they use.
+using phylo_amd::sub_model;
+#endif
+
 #endif

@@ -1,5 +1,5 @@
 // tree.cpp -- see tree.h.
-#include "tree.h"
+#include "phylo_tree.h"
 
 #include <cstdio>
 #include <cstdlib>
@@ -18,19 +18,24 @@ int tree::new_node() {
     return (int)nodes_.size() - 1;
 }
 
-void tree::tree_file_parser(std::istream& infile) { tree_file_parser(infile, {}); }
+void tree::tree_file_parser(std::istream& infile) { tree_file_parser(infile, {}, false); }
 
 void tree::tree_file_parser(std::istream& infile, const std::map<std::string, std::string>& tr) {
+    tree_file_parser(infile, tr, false);
+}
+
+void tree::tree_file_parser(std::istream& infile, const std::map<std::string, std::string>& tr, bool keep_extra) {
     clear();
     int present = root_;
-    std::string label, blen;
+    std::string label, extra, blen;
     char mode = 's', c;
     auto flush = [&](int v) {
-        if (!label.empty()) {
+        if (!label.empty() || !extra.empty()) {
             auto it = tr.find(label);
-            nodes_[v].label = (it != tr.end()) ? it->second : label;
+            nodes_[v].label = ((it != tr.end()) ? it->second : label) + extra;
             nodes_[v].has_label = true;
             label.clear();
+            extra.clear();
         }
         if (!blen.empty()) {
             nodes_[v].branchlength = atof(blen.c_str());
@@ -42,7 +47,14 @@ void tree::tree_file_parser(std::istream& infile, const std::map<std::string, st
         if (c == '[') {                                  // comments, possibly nested
             ok = (bool)(infile >> c);
             if (!ok) break;
-            if (c != ']') {
+            if (c == '&' && keep_extra) {                // "[&...]": kept, joins the label
+                extra += '[';
+                while (ok && c != ']') {
+                    extra += c;
+                    ok = (bool)(infile >> c);
+                }
+                if (ok) extra += ']';
+            } else if (c != ']') {
                 int depth = 1;
                 while (ok && depth > 0) {
                     if (c == '[') ++depth;
@@ -58,6 +70,7 @@ void tree::tree_file_parser(std::istream& infile, const std::map<std::string, st
             nodes_[k].parent = present;
             present = k;
             label.clear();
+            extra.clear();
         } else if (c == ',') {
             mode = 'l';
             flush(present);
@@ -146,6 +159,79 @@ void tree::write_subtree(std::string& out, int v, bool br, bool int_label) const
         snprintf(buf, sizeof buf, ":%f", n.branchlength);
         out += buf;
     }
+}
+
+template <class TipName>
+void tree::write_subtree_with(std::string& out, int v, bool br, bool int_label, TipName tip_name) const {
+    const node& n = nodes_[v];
+    if (n.left >= 0) {
+        out += '(';
+        write_subtree_with(out, n.left, br, int_label, tip_name);
+    } else {
+        out += tip_name(v);
+    }
+    if (n.right >= 0) {
+        out += ',';
+        write_subtree_with(out, n.right, br, int_label, tip_name);
+        out += ')';
+        if (int_label && n.has_label) out += n.label;
+    }
+    if (v != root_ && br) {
+        char buf[64];
+        snprintf(buf, sizeof buf, ":%f", n.branchlength);
+        out += buf;
+    }
+}
+
+namespace {
+// A label without its "[...]" annotation, and the annotation alone.
+std::string bare_name(const std::string& label) { return label.substr(0, label.find('[')); }
+std::string annotation(const std::string& label) {
+    const size_t k = label.find('[');
+    return k == std::string::npos ? std::string() : label.substr(k);
+}
+}  // namespace
+
+void tree::print_nexus_tree_intro(std::ostream& out, std::map<std::string, std::string>& translate_taxa) const {
+    out << "#NEXUS" << std::endl << "Begin taxa;" << std::endl;
+    out << "\tDimensions ntax=" << n_tips() << ";" << std::endl << "\tTaxlabels" << std::endl;
+    std::vector<std::pair<std::string, std::string>> rows;       // (key, name) in output order
+    if (translate_taxa.empty()) {
+        int k = 0;
+        for (int v : tips()) {
+            const std::string name = nodes_[v].has_label ? bare_name(nodes_[v].label) : std::string();
+            rows.emplace_back(std::to_string(++k), name);
+            translate_taxa[rows.back().first] = name;
+        }
+    } else {
+        for (const auto& kv : translate_taxa) rows.emplace_back(kv.first, kv.second);
+    }
+    for (const auto& r : rows) out << "\t\t" << r.second << std::endl;
+    out << ";" << std::endl << "End;" << std::endl << std::endl;
+    out << "Begin trees;" << std::endl << "\tTranslate" << std::endl;
+    for (size_t i = 0; i < rows.size(); ++i) {
+        out << "\t\t" << rows[i].first << ' ' << rows[i].second;
+        if (i + 1 < rows.size()) out << ',' << std::endl;
+    }
+    out << std::endl << ";" << std::endl;
+}
+
+void tree::print_tree_to_nexus(std::ostream& out, const std::string& name, bool br, bool int_label,
+                               const std::map<std::string, std::string>& translate_taxa) const {
+    int counter = 1;
+    std::string s;
+    write_subtree_with(s, root_, br, int_label, [&](int v) {
+        const node& n = nodes_[v];
+        if (translate_taxa.empty())
+            return std::to_string(counter++) + (n.has_label ? annotation(n.label) : std::string());
+        if (!n.has_label) return std::string();
+        const std::string bare = bare_name(n.label);
+        std::string key;
+        for (const auto& kv : translate_taxa)
+            if (kv.second == bare) { key = kv.first; break; }
+        return key + annotation(n.label);
+    });
+    out << "tree " << name << " = " << s << ";" << std::endl;
 }
 
 std::string tree::newick(bool br, bool int_label) const {

@@ -195,6 +195,46 @@ def main():
         tm.append({"arg": arg, "pairs": pairs, "out": [float(x) for x in out.split()]})
     write(os.path.join(HERE, "time_mod_length.json"), json.dumps(tm) + "\n")
 
+    # NEXUS tree output (print_nexus_tree_intro / print_tree_to_nexus) and
+    # trees read with "[&...]" annotations kept
+    d = os.path.join(HERE, "nexus")
+    trees = ("((a:0.1,b:0.2)[&x=1]:0.05,(c[&y=2]:0.3,d:0.4)e:0.1,f:0.7)r;\n"
+             "(t2:1,(t1:2,t3:3):0.5);\n"
+             "('quoted name':0.25,(p[comment]:1e-3,q:2)[&support=0.9]90:0.125,(r,s));\n")
+    write(os.path.join(d, "input.nwk"), trees)
+    write(os.path.join(d, "expected_nexus.txt"), run("nexus", os.path.join(d, "input.nwk")))
+    write(os.path.join(d, "expected_parsex.txt"), run("parsex", os.path.join(d, "input.nwk")))
+    nex_example = os.path.join(REF_EXAMPLES, "tree_file.tree") if os.path.isdir(REF_EXAMPLES) else None
+    if nex_example:
+        write(os.path.join(d, "expected_example_nexus.txt"), run("nexus", nex_example))
+
+    # the reference's own treeator on its example files: stdout of each command
+    # (the binding test runs the same commands through treeator built against
+    # the host classes)
+    tre = os.path.join(ROOT, "oracle", "_ref", "treeator_ref")
+    d = os.path.join(HERE, "treeator_cli")
+    ex = os.path.join(HERE, "example_files")
+    jc = ["-a", "dna", "-t", "tree_file.tree", "-l", "-N", "--tr", "-m", "0,0,0,0,0,0", "-P", "1.3333333333333333",
+          "-F", "0.25,0.25,0.25,0.25"]
+    cmds = {
+        "jc": jc + ["alignment_file.fst"],
+        "k2": ["-a", "dna", "-t", "tree_file.tree", "-l", "-N", "--tr", "-m", "0,1,0,0,1,0", "-P", "1.0,0.25",
+               "-F", "0.1,0.2,0.3,0.4", "alignment_file.fst"],
+        "nodes": jc + ["--get_state_at_nodes", "alignment_file.fst"],
+        "nexus": jc + ["--output", "nexus", "alignment_file.fst"],
+        "timerate": jc + ["-U", "2:0.01", "alignment_file.fst"],
+        "clade": jc + ["-V", "3.0:AY700180_Entoloma_prunuloides,AY634279_Pluteus_romellii", "alignment_file.fst"],
+        "nonrev": ["-a", "dna", "-t", "tree_file.tree", "-l", "-N", "-m", ",".join(["0"] * 12), "-P", "0.3",
+                   "alignment_file.fst"],
+    }
+    os.makedirs(d, exist_ok=True)
+    for name, args in cmds.items():
+        out = subprocess.run([tre, *args], capture_output=True, text=True, timeout=120, cwd=ex)
+        if out.returncode != 0:
+            raise RuntimeError(f"treeator {name}: rc={out.returncode} {out.stderr[-300:]}")
+        write(os.path.join(d, name + ".out"), out.stdout)
+    write(os.path.join(d, "commands.json"), json.dumps(cmds, indent=1) + "\n")
+
     # the rate models that -V (clade) and time-rate arguments build, edge cases
     # of their token grammar included (rate_model.cpp:129-176)
     ra = []

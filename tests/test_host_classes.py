@@ -63,3 +63,16 @@ def test_rate_arguments_build_the_reference_models():
     assert len(cases) >= 30
     for c in cases:
         assert drive("ratearg", c["kind"], c["arg"]) == c["out"], (c["kind"], c["arg"])
+
+
+@pytest.mark.parametrize("mode,src,want", [
+    ("nexus", "input.nwk", "expected_nexus.txt"),
+    ("parsex", "input.nwk", "expected_parsex.txt"),
+    ("nexus", "../example_files/tree_file.tree", "expected_example_nexus.txt"),
+])
+def test_nexus_output_and_annotations_match_reference(mode, src, want):
+    """print_nexus_tree_intro / print_tree_to_nexus / print_nexus_tree_end
+    with an empty and then a filled Translate table, and trees read with
+    "[&...]" annotations kept, against the compiled reference's output."""
+    d = os.path.join(GOLDEN, "nexus")
+    assert drive(mode, os.path.join(d, src)) == open(os.path.join(d, want)).read()
