@@ -16,6 +16,9 @@
 //   tmbl     TIMEARG B1 D1 [B2 D2 ...]              get_time_mod_branch_length dumps
 //   ratearg  clade|time ARG                        the rate_model pars_clade_rates_arg /
 //            pars_rates_and_times_arg build, through its getters
+//   matrix   FORMAT FILE                            rows the reference's file_parser + matrix_parser
+//            read (fasta | phylip | nexus, DNA alphabet), state sets as integers
+//   ntrees   FILE                                   Translate table and trees of a NEXUS TREES block
 //   nexus    TREE                                   print_nexus_tree_intro / print_tree_to_nexus /
 //            print_nexus_tree_end, empty then filled Translate table
 //   parsex   TREE                                   print_newick of trees read with "[&...]" kept
@@ -37,6 +40,7 @@
 #include <vector>
 
 #include "simpleML.h"
+#include "file_parser.h"
 #include "matrix_parser.h"
 #include "rate_model.h"
 #include "sub_model.h"
@@ -203,6 +207,52 @@ int main(int argc, char** argv) {
             model.set_P_matrix(atof(argv[a]));
             for (unsigned int i = 0; i < S; ++i)
                 for (unsigned int j = 0; j < S; ++j) printf("%.17g%c", model.get_P_value(i, j), j + 1 < S ? ' ' : '\n');
+        }
+        return 0;
+    }
+    if (mode == "matrix") {
+        // the reference's own readers as treeator drives them (treeator.cpp:438-455)
+        std::ifstream f(argv[3]);
+        file_parser input(&f);
+        input.set_file_type(argv[2]);
+        if (input.test_file_type("nexus") && !input.move_to_next_X_block(nexus_block::DATA)) return 5;
+        map<char, bitset<SIZE> > alpha;
+        alphabet::set_alphabet_dna(alpha);
+        partitions regions;
+        regions.add_alphabet("first", alpha);
+        regions.add_partition(0, 0, "default", "first");
+        vector<character_vector> rows;
+        matrix_parser parser(*input.file_stream, rows, regions, input.get_file_type());
+        parser.pars();
+        for (auto& r : rows) {
+            printf(">%s\n", r.get_taxon().c_str());
+            for (unsigned int k = 0; k < r.n_char(); ++k) printf("%s%lu", k ? "," : "", r.get_character(k).to_ulong());
+            printf("\n");
+        }
+        return 0;
+    }
+    if (mode == "ntrees") {
+        // NEXUS TREES block: Translate table, then every tree (treeator.cpp:490-530)
+        std::ifstream f(argv[2]);
+        file_parser input(&f);
+        input.set_file_type("nexus");
+        if (!input.move_to_next_X_block(nexus_block::TREES)) return 5;
+        map<string, string> taxa_trans;
+        char cmd = input.read_next_nexus_command();
+        if (cmd == nexus_command::TRANSLATE) {
+            input.read_translate_parameters(taxa_trans);
+            cmd = input.read_next_nexus_command();
+        }
+        for (auto& kv : taxa_trans) printf("translate [%s] [%s]\n", kv.first.c_str(), kv.second.c_str());
+        unsigned int n = 0;
+        while (true) {
+            if (n != 0) cmd = input.read_next_nexus_command();
+            if (!(cmd == nexus_command::TREE && input.move_to_start_of_tree())) break;
+            tree t;
+            t.tree_file_parser(*input.file_stream, taxa_trans, false);
+            if (t.empty()) break;
+            ++n;
+            t.print_newick();
         }
         return 0;
     }

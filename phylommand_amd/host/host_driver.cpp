@@ -9,9 +9,15 @@
 //   nniscore TREE FASTA ALPHA TR SPEC PARAMS FREQS MODE     NNI neighbourhood lnL
 //   parse    TREE          print_newick of every tree in the file
 //   parsex   TREE          the same with "[&...]" annotations kept in the labels
+//   matrix   FORMAT FILE   rows read by the fasta / relaxed-phylip / NEXUS DATA readers (DNA)
+//   ntrees   FILE          Translate table and trees of a NEXUS TREES block
 //   nexus    TREE          NEXUS writer: intro, tree, end (empty, then filled Translate table)
 //   tmbl     TIMEARG B D [B D ...]
 //   ratearg  clade|time ARG   the rate model a -V / time-rate argument builds
+//   objective TREE FASTA ALPHA TR SPEC PARAMS FREQS FIXED FIXFREQ FIXEDX TIMEARG CLADEARG METHOD [X ...]
+//            treeator's fit vector (x0, bounds) and its objective at each X
+//   optimize TREE FASTA ALPHA TR SPEC PARAMS FREQS FIXED FIXFREQ FIXEDX TIMEARG CLADEARG METHOD
+//            optimize_parameters: final lnL, evaluations, rates, frequencies, rate model
 //   pmatrix  S TR SPEC PARAMS FREQS T [T ...]
 //   simulate TREE ALPHA TR SPEC PARAMS FREQS NCHAR SEED    fasta on stdout
 // ALPHA is dna | protein | binary | an alphabet file; lists are comma
@@ -23,6 +29,7 @@
 #include <sstream>
 
 #include "io.h"
+#include "optimize.h"
 #include "rate_model.h"
 #include "simpleML.h"
 #include "sub_model.h"
@@ -171,6 +178,88 @@ int main(int argc, char** argv) {
             for (auto& x : out) x.print_newick();
             break;
         }
+        return 0;
+    }
+    if (mode == "objective" || mode == "optimize") {
+        // treeator's model fit: the parameter vector and its objective, or the
+        // fit itself (FP64 partials, eigen / Pade P)
+        if (argc < 15) return 2;
+        Data d;
+        load_data(argv[3], argv[4], d);
+        sub_model model(d.n_states, atoi(argv[5]) != 0);
+        if (!setup_model(model, argv[6], argv[7], argv[8])) { std::cerr << "model setup failed" << std::endl; return 3; }
+        std::stringstream ts(slurp(argv[2]));
+        simpleML t;
+        t.tree_file_parser(ts);
+        t.set_precision_fp64(true);
+        rate_model rate_mod;
+        std::vector<std::string> sets;
+        const std::string targ = argv[12], carg = argv[13];
+        if (targ != "-" && !rate_mod.pars_rates_and_times_arg(targ.c_str())) return 4;
+        if (carg != "-" && !rate_mod.pars_clade_rates_arg(carg.c_str(), sets)) return 4;
+        if (!sets.empty()) t.add_taxon_sets(sets);
+        fill_chars(t, d, model, d.rows.empty() ? 0 : d.rows[0].n_char());
+        std::vector<unsigned int> fixed, fixed_extra;
+        for (double v : dlist(argv[9])) fixed.push_back((unsigned int)v);
+        for (double v : dlist(argv[11])) fixed_extra.push_back((unsigned int)v);
+        const bool fix_freq = atoi(argv[10]) != 0;
+        const char method = argv[14][0];
+        const auto print = [](const char* name, const std::vector<double>& v) {
+            printf("%s", name);
+            for (double x : v) printf(" %.17g", x);
+            printf("\n");
+        };
+        if (mode == "objective") {
+            model_objective obj(t, model, rate_mod, fixed, fix_freq, fixed_extra, method);
+            print("x0", obj.x0);
+            print("lo", obj.lo);
+            print("hi", obj.hi);
+            for (int a = 15; a < argc; ++a) printf("f %.17g\n", obj(dlist(argv[a])));
+            return 0;
+        }
+        nm_options opt;
+        opt.max_evals = 20000;
+        opt.ftol_abs = 1e-7;
+        int evals = 0;
+        const double l = optimize_parameters(t, model, rate_mod, fixed, fix_freq, fixed_extra, method, opt, &evals);
+        printf("lnl %.17g\nevals %d\n", l, evals);
+        std::vector<double> rates, freqs;
+        for (unsigned int i = 0; i < model.get_n_rates(); ++i) rates.push_back(model.get_rate(i));
+        for (unsigned int i = 0; i < model.get_n_states(); ++i) freqs.push_back(model.get_freq(i));
+        print("rates", rates);
+        print("freqs", freqs);
+        std::vector<double> cuts, time_rates;
+        for (unsigned int i = 0; i < rate_mod.get_n_cut_offs(); ++i) {
+            cuts.push_back(rate_mod.get_cut_off(i));
+            time_rates.push_back(rate_mod.get_rate_in_time(i));
+        }
+        print("cut_offs", cuts);
+        print("time_rates", time_rates);
+        return 0;
+    }
+    if (mode == "matrix") {
+        alphabet_map alpha;
+        alphabet::set_alphabet_dna(alpha);
+        std::ifstream f(argv[3]);
+        std::vector<character_vector> rows;
+        const std::string fmt = argv[2];
+        if (fmt == "fasta") parse_fasta(f, alpha, rows);
+        else if (fmt == "phylip") parse_relaxed_phylip(f, alpha, rows);
+        else if (!parse_nexus_data(f, alpha, rows)) return 5;
+        for (const auto& r : rows) {
+            printf(">%s\n", r.get_taxon().c_str());
+            for (unsigned int k = 0; k < r.n_char(); ++k) printf("%s%lu", k ? "," : "", r.character(k).to_ulong());
+            printf("\n");
+        }
+        return 0;
+    }
+    if (mode == "ntrees") {
+        std::ifstream f(argv[2]);
+        nexus_tree_reader reader(f);
+        if (!reader.found()) return 5;
+        for (const auto& kv : reader.translation()) printf("translate [%s] [%s]\n", kv.first.c_str(), kv.second.c_str());
+        tree t;
+        while (reader.next(t)) t.print_newick();
         return 0;
     }
     if (mode == "nexus" || mode == "parsex") {

@@ -55,8 +55,35 @@ char translate_bitset(const state_set& s, const alphabet_map& a);
 // empty state sets; call set_empty_to to turn them into "any state".
 void parse_fasta(std::istream& in, const alphabet_map& a, std::vector<character_vector>& out);
 // matrix_parser::pars_relaxed_phylip (matrix_parser.cpp:27-91): a
-// "n_taxa n_char" header, then "name sequence" rows.
+// "n_taxa n_char" header, then "name sequence" rows, one per line; only
+// letters of the alphabet are kept and a row is kept when its line ends.
 void parse_relaxed_phylip(std::istream& in, const alphabet_map& a, std::vector<character_vector>& out);
+// A NEXUS DATA block: file_parser::move_to_next_X_block(DATA) then
+// matrix_parser::pars_nexus (file_parser.cpp:62-100, matrix_parser.cpp:140-225):
+// commands up to MATRIX, then rows as in relaxed phylip up to ';'.
+// Returns false when the stream has no DATA block.
+bool parse_nexus_data(std::istream& in, const alphabet_map& a, std::vector<character_vector>& out);
+
+class tree;
+// The trees of a NEXUS TREES block, read as treeator does (treeator.cpp:490-530
+// with file_parser.cpp:102-283): an optional Translate table first, then one
+// tree per TREE command, tip labels translated through the table.
+class nexus_tree_reader {
+  public:
+    explicit nexus_tree_reader(std::istream& in);
+    bool found() const { return ok_; }                     // a TREES block exists
+    const std::map<std::string, std::string>& translation() const { return translation_; }
+    // the next tree into t; false after the last
+    bool next(tree& t);
+
+  private:
+    std::string next_command();
+    void read_translate();
+    std::istream& in_;
+    std::map<std::string, std::string> translation_;
+    std::string pending_;
+    bool ok_ = false, first_ = true;
+};
 
 // Number of states the data need (treeator.cpp:555-558).
 unsigned int n_states_of(const std::vector<character_vector>& rows);

@@ -281,28 +281,30 @@ namespace {
 bool contains(const std::vector<unsigned int>& v, unsigned int x) { return std::find(v.begin(), v.end(), x) != v.end(); }
 }  // namespace
 
-double optimize_parameters(simpleML& t, sub_model& model, rate_model& rate_mod, const std::vector<unsigned int>& fixed,
-                           bool fixed_freq, const std::vector<unsigned int>& fixed_extra, char method,
-                           const nm_options& opt, int* evals) {
-    const double big = 1e6;
-    std::vector<double> x0, lo, hi;
+model_objective::model_objective(simpleML& t_, sub_model& model_, rate_model& rate_mod_,
+                                 const std::vector<unsigned int>& fixed_, bool fixed_freq,
+                                 const std::vector<unsigned int>& fixed_extra_, char method_)
+    : t(t_), model(model_), rate_mod(rate_mod_), fixed(fixed_), fixed_extra(fixed_extra_),
+      free_freq(model_.is_tr() && !fixed_freq), method(method_) {
+    const double big = 1e6;     // the reference's DBL_MAX, kept finite for the simplex arithmetic
     for (unsigned int i = 0; i < model.get_n_rates(); ++i)
         if (!contains(fixed, i)) { x0.push_back(model.get_rate(i)); lo.push_back(0.0); hi.push_back(big); }
-    const size_t start_freq = x0.size();
-    const bool free_freq = model.is_tr() && !fixed_freq;
+    start_freq = x0.size();
     if (free_freq)
         for (unsigned int i = 0; i + 1 < model.get_n_states(); ++i) {
             x0.push_back(model.get_freq(i));
             lo.push_back(0.0);
             hi.push_back(1.0);
         }
-    const size_t start_extra = x0.size();
+    start_extra = x0.size();
     if (rate_mod.get_n_rates())
         for (unsigned int i = 0; i < rate_mod.get_n_parameters(); ++i) {
             if (contains(fixed_extra, i)) continue;
             lo.push_back(0.0);
             if ((method == 't' || method == 'S') && i < rate_mod.get_n_cut_offs()) {
                 hi.push_back(t.longest_to_tip());
+                // the reference starts every cut-off at the first one's value
+                // (get_cut_off(0), treeator.cpp:774)
                 x0.push_back(rate_mod.get_cut_off(0));
             } else {
                 hi.push_back(big);
@@ -310,27 +312,36 @@ double optimize_parameters(simpleML& t, sub_model& model, rate_model& rate_mod, 
                 else x0.push_back(rate_mod.get_rate_clade(i));
             }
         }
-    if (x0.empty()) return t.calculate_log_likelihood(model, rate_mod);
-    // change_non_fixed (treeator.cpp:1152-1182); invalid sets score -inf
-    auto apply = [&](const std::vector<double>& x) {
-        size_t j = 0;
-        for (unsigned int i = 0; i < model.get_n_rates(); ++i)
-            if (!contains(fixed, i) && !model.set_rate(i, x[j++])) return false;
-        if (free_freq)
-            for (unsigned int i = 0; i + 1 < model.get_n_states(); ++i)
-                if (!model.set_freq(i, x[start_freq + i])) return false;
-        j = start_extra;
-        for (unsigned int i = 0; i < rate_mod.get_n_parameters() && rate_mod.get_n_rates(); ++i)
-            if (!contains(fixed_extra, i)) rate_mod.set_parameter(i, (float)x[j++]);
-        return true;
-    };
-    auto f = [&](const std::vector<double>& x) {
-        if (!apply(x)) return -std::numeric_limits<double>::infinity();
-        const double l = t.calculate_log_likelihood(model, rate_mod);
-        return (l == 0.0 || !std::isfinite(l)) ? -std::numeric_limits<double>::infinity() : l;
-    };
-    const nm_result r = nelder_mead(f, x0, lo, hi, opt);
-    apply(r.x);
+}
+
+bool model_objective::apply(const std::vector<double>& x) {
+    size_t j = 0;
+    for (unsigned int i = 0; i < model.get_n_rates(); ++i)
+        if (!contains(fixed, i) && !model.set_rate(i, x[j++])) return false;
+    if (free_freq)
+        for (unsigned int i = 0; i + 1 < model.get_n_states(); ++i)
+            if (!model.set_freq(i, x[start_freq + i])) return false;
+    j = start_extra;
+    for (unsigned int i = 0; i < rate_mod.get_n_parameters() && rate_mod.get_n_rates(); ++i)
+        if (!contains(fixed_extra, i)) rate_mod.set_parameter(i, (float)x[j++]);
+    return true;
+}
+
+double model_objective::operator()(const std::vector<double>& x) {
+    const double ninf = -std::numeric_limits<double>::infinity();
+    if (!apply(x)) return ninf;
+    if ((method == 't' || method == 'S') && rate_mod.get_n_time_rates() < 1) return ninf;
+    const double l = t.calculate_log_likelihood(model, rate_mod);
+    return (l == 0.0 || !std::isfinite(l)) ? ninf : l;
+}
+
+double optimize_parameters(simpleML& t, sub_model& model, rate_model& rate_mod, const std::vector<unsigned int>& fixed,
+                           bool fixed_freq, const std::vector<unsigned int>& fixed_extra, char method,
+                           const nm_options& opt, int* evals) {
+    model_objective obj(t, model, rate_mod, fixed, fixed_freq, fixed_extra, method);
+    if (obj.x0.empty()) return t.calculate_log_likelihood(model, rate_mod);
+    const nm_result r = nelder_mead([&](const std::vector<double>& x) { return obj(x); }, obj.x0, obj.lo, obj.hi, opt);
+    obj.apply(r.x);
     if (evals) *evals = r.evals;
     return t.calculate_log_likelihood(model, rate_mod);
 }

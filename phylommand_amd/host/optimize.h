@@ -67,6 +67,30 @@ class simpleML;
 class sub_model;
 class rate_model;
 
+// treeator's parameter vector for the model fit (treeator.cpp:747-785) and
+// its objective (opt_function, opt_rate_in_time, opt_rate_for_clades_function
+// through change_non_fixed, treeator.cpp:1122-1182).  Built from the model's
+// current values, which are the starting point x0.
+struct model_objective {
+    model_objective(simpleML& t, sub_model& model, rate_model& rate_mod, const std::vector<unsigned int>& fixed,
+                    bool fixed_freq, const std::vector<unsigned int>& fixed_extra, char method);
+    // change_non_fixed: writes x into the model and rate model; false when
+    // the model rejects a value (a negative rate, frequencies past 1)
+    bool apply(const std::vector<double>& x);
+    // lnL at x, or -infinity when x is invalid or the likelihood is not
+    // finite (the reference's objective returns 0.0 there)
+    double operator()(const std::vector<double>& x);
+
+    simpleML& t;
+    sub_model& model;
+    rate_model& rate_mod;
+    std::vector<unsigned int> fixed, fixed_extra;
+    bool free_freq;
+    char method;
+    std::vector<double> x0, lo, hi;
+    size_t start_freq = 0, start_extra = 0;
+};
+
 // treeator's model-parameter optimisation (treeator.cpp:738-911): the rates
 // not in `fixed`, then (reversible models, frequencies not fixed) the first
 // n-1 frequencies in [0, 1], then the rate-model parameters not in

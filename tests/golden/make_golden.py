@@ -208,6 +208,18 @@ def main():
     if nex_example:
         write(os.path.join(d, "expected_example_nexus.txt"), run("nexus", nex_example))
 
+    # the readers treeator uses for data and trees: fasta, relaxed phylip and
+    # NEXUS DATA through file_parser + matrix_parser, NEXUS TREES with a
+    # Translate table (inputs hand-written under formats/, quirks included:
+    # a tab-indented MATRIX is not found, quoted Translate names run on)
+    d = os.path.join(HERE, "formats")
+    fm = []
+    for args in (("matrix", "nexus", "data.nex"), ("matrix", "nexus", "quirks.nex"), ("matrix", "phylip", "data.phy"),
+                 ("matrix", "fasta", "../example_files/alignment_file.fst"), ("ntrees", "data.nex"),
+                 ("ntrees", "quirks.nex")):
+        fm.append({"args": list(args), "out": run(*args[:-1], os.path.join(d, args[-1]))})
+    write(os.path.join(d, "expected.json"), json.dumps(fm, indent=1) + "\n")
+
     # the reference's own treeator on its example files: stdout of each command
     # (the binding test runs the same commands through treeator built against
     # the host classes)

@@ -76,3 +76,15 @@ def test_nexus_output_and_annotations_match_reference(mode, src, want):
     "[&...]" annotations kept, against the compiled reference's output."""
     d = os.path.join(GOLDEN, "nexus")
     assert drive(mode, os.path.join(d, src)) == open(os.path.join(d, want)).read()
+
+
+def test_data_and_tree_readers_match_reference():
+    """fasta, relaxed phylip and NEXUS DATA rows, and NEXUS TREES (Translate
+    table, comments, several trees), against what the reference's
+    file_parser + matrix_parser read from the same files."""
+    d = os.path.join(GOLDEN, "formats")
+    cases = json.load(open(os.path.join(d, "expected.json")))
+    assert len(cases) == 6
+    for c in cases:
+        *mode, name = c["args"]
+        assert drive(*mode, os.path.join(d, name)) == c["out"], c["args"]
