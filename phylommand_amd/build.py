@@ -58,6 +58,19 @@ def build_host():
     os.replace(tmp, HOST_DRIVER)
 
 
+def build_host_sanitized(out_dir):
+    """host_driver with the host classes compiled in, under AddressSanitizer
+    and UndefinedBehaviorSanitizer (host C++ only, g++; it loads the normal
+    libphylo_hip.so), for the CPU sanitizer test.  Returns its path."""
+    os.makedirs(out_dir, exist_ok=True)
+    exe = os.path.join(out_dir, "host_driver_san")
+    srcs = [os.path.join(HOST, f) for f in ["host_driver.cpp", *HOST_SRCS]]
+    run(["g++", "-std=c++17", "-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+         "-fno-sanitize-recover=all", "-I" + os.path.join(ROOT, "include"), "-I" + HOST, "-o", exe, *srcs,
+         "-L" + HERE, "-lphylo_hip", "-Wl,-rpath," + HERE])
+    return exe
+
+
 TREEATOR_DIR = os.path.join(HOST, "_treeator")
 REF_SRC = os.environ.get("PHYLOMMAND_SRC", "/root/reference/src")
 
