@@ -47,6 +47,16 @@ constexpr int kMaxStates = 32;
 constexpr double kLn2 = 0.69314718055994530942;
 constexpr int kRootCatOff = 32;   // roots[]: per-state weights [0, 32), category weights after
 
+// Workgroups per CU below which a whole-tree launch has too few waves to
+// hide its dependent chain: every site block is then walked as subtree
+// pieces plus the top program (the tail split applied to all blocks), which
+// multiplies the independent waves by the number of pieces.
+constexpr int kSplitAllBelowPerCu = 4;
+
+// Site groups per wave of the MFMA walker: two independent MFMA chains per
+// wave share each P operand load and overlap their memory latency.
+constexpr int kMfmaGroups = 2;
+
 enum SrcKind : int { SRC_TIP = 0, SRC_CLV = 1, SRC_CARRY_A = 2, SRC_CARRY_B = 3, SRC_NONE = 4 };
 enum OpFlags : int { OP_ROOT = 1, OP_TO_B = 2 };
 
@@ -1060,6 +1070,7 @@ struct phylo_ctx {
     // tail split of whole-tree DNA walks (build_split_programs): split_K chunk
     // programs from program id split_base[full ? 0 : 1], then one top program
     int split_K = 0, split_base[2] = {0, 0};
+    bool split_all = false;                               // every block walks the pieces (too few blocks)
     bool tail_split = true;                               // phylo_set_tail_split
     int n_cu = 0;                                         // compute units of the device
     bool run_full = false;                                // the whole-tree launch stores every partial
@@ -1294,12 +1305,12 @@ void drop_unread_stores(std::vector<Op>& ops, size_t begin, size_t end) {
 // storing and the lean evaluation, split_K chunk programs and one top program
 // to c->ops / c->prog_begin; split_K stays 0 when no cut keeps the top
 // program short (a caterpillar tree, say) or the tree is small.
-void build_split_programs(phylo_ctx* c) {
+bool build_split_programs(phylo_ctx* c, int max_chunks, bool split_all) {
     c->split_K = 0;
     const int nt = c->n_taxa, nn = c->n_nodes, root = c->root;
     const std::vector<int> post = internal_postorder(c, c->left, c->right);
     const int ni = (int)post.size();
-    if (ni < 16) return;
+    if (ni < 16) return false;
     std::vector<int> size(nn, 0), par(nn, -1);
     for (int i = 0; i < ni; ++i) {
         const int v = post[i], l = c->left[v], r = c->right[v];
@@ -1328,21 +1339,31 @@ void build_split_programs(phylo_ctx* c) {
     double best_cost = 1e30;
     std::vector<std::vector<int>> chunks;
     std::vector<int> top;
-    for (int d : {16, 12, 10, 8, 6, 5, 4, 3, 2}) {
-        const int L = std::max(1, ni / d);
+    // piece sizes to try: fractions of the tree, and (so that few large
+    // pieces are possible) the sizes of the subtrees near the root
+    std::vector<int> Ls;
+    for (int d : {16, 12, 10, 8, 6, 5, 4, 3, 2}) Ls.push_back(std::max(1, ni / d));
+    for (int v : {c->left[root], c->right[root]})
+        if (v >= nt) {
+            Ls.push_back(size[v]);
+            for (int w : {c->left[v], c->right[v]})
+                if (w >= nt) Ls.push_back(size[w]);
+        }
+    for (int L : Ls) {
         if (size[root] <= L) continue;
         cut(L, chunks, top);
         size_t big = 0;
         for (const auto& ch : chunks) big = std::max(big, ch.size());
-        if (chunks.size() < 2 || top.size() > 16) continue;
-        // the top program runs alone, about 0.7 us per op; a chunk shares a
-        // CU with main blocks and costs little more as it grows (C2 tree:
-        // top 2 / chunks <= 56 ops 0.137 ms, top 6 / <= 42 0.139, top 9 /
-        // <= 20 0.141, top 15 / <= 15 0.145)
-        const double cost = (double)top.size() + (double)big / 40.0;
+        if (chunks.size() < 2 || top.size() > 16 || (int)chunks.size() > max_chunks) continue;
+        // Tail split: the top program runs alone, about 0.7 us per op; a
+        // chunk shares a CU with main blocks and costs little more as it
+        // grows (C2 tree: top 2 / chunks <= 56 ops 0.137 ms, top 6 / <= 42
+        // 0.139, top 9 / <= 20 0.141, top 15 / <= 15 0.145).  Every block
+        // split: the longest chunk and then the top program form the chain.
+        const double cost = split_all ? (double)big + (double)top.size() : (double)top.size() + (double)big / 40.0;
         if (cost < best_cost) { best_cost = cost; best_L = L; }
     }
-    if (best_L < 0) return;
+    if (best_L < 0) return false;
     cut(best_L, chunks, top);
     const auto emit = [&](const std::vector<int>& nodes, bool store_all) {
         for (size_t i = 0; i < nodes.size(); ++i) {
@@ -1371,6 +1392,48 @@ void build_split_programs(phylo_ctx* c) {
         for (const auto& ch : chunks) emit(ch, mode == 0);
         emit(top, mode == 0);
     }
+    return true;
+}
+
+// Workgroups of the whole-tree walker that one CU holds at once, for this
+// context's alphabet, precision and rate categories (the occupancy API on the
+// kernel instance the launch uses); 0 if unknown.
+template <typename T, int C>
+int resident_dna(const phylo_ctx*) {
+    int n = 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, prune4_kernel<T, C, true>, 64 * C, 0) == hipSuccess ? n : 0;
+}
+template <typename T, int C, int V>
+int resident_wide(const phylo_ctx*) {
+    int n = 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, prune_mfma_kernel<T, C, V, kMfmaGroups, true>, 64 * C, 0) ==
+                   hipSuccess ? n : 0;
+}
+template <typename T, int C>
+int resident_wide_v(const phylo_ctx* c) {
+    switch (mfma_slots(c->S)) {
+    case 2: return resident_wide<T, C, 2>(c);
+    case 3: return resident_wide<T, C, 3>(c);
+    case 4: return resident_wide<T, C, 4>(c);
+    case 5: return resident_wide<T, C, 5>(c);
+    case 6: return resident_wide<T, C, 6>(c);
+    case 7: return resident_wide<T, C, 7>(c);
+    default: return resident_wide<T, C, 8>(c);
+    }
+}
+template <typename T>
+int resident_per_cu_t(const phylo_ctx* c) {
+    const bool wide = c->S > 4;
+    switch (c->C) {
+    case 1: return wide ? resident_wide_v<T, 1>(c) : resident_dna<T, 1>(c);
+    case 2: return wide ? resident_wide_v<T, 2>(c) : resident_dna<T, 2>(c);
+    case 4: return wide ? resident_wide_v<T, 4>(c) : resident_dna<T, 4>(c);
+    case 8: return wide ? resident_wide_v<T, 8>(c) : resident_dna<T, 8>(c);
+    default: return 0;
+    }
+}
+int resident_per_cu(const phylo_ctx* c) {
+    return c->prec == PHYLO_PREC_FP64 ? resident_per_cu_t<double>(c) : resident_per_cu_t<float>(c);
 }
 
 int compile_full_program(phylo_ctx* c) {
@@ -1381,7 +1444,25 @@ int compile_full_program(phylo_ctx* c) {
     drop_unread_stores(c->ops, n, 2 * n);
     c->prog_begin = {0, (int)n, (int)(2 * n)};
     c->n_slots = c->n_nodes - c->n_taxa;
-    build_split_programs(c);
+    // When every site block is split (too few blocks to fill the CUs), its
+    // pieces run as K workgroups per block: K is capped so that all of them
+    // are resident at once, or the pieces would run in rounds and the walk
+    // would take no less than the whole-tree chain.  A tail beside full
+    // rounds needs no cap.
+    c->split_all = false;
+    const int nblk = c->m_pad / (uses_mfma(c) ? 16 * kMfmaGroups : 64);
+    if (c->n_cu > 0 && nblk < kSplitAllBelowPerCu * c->n_cu) {
+        const int per_cu = resident_per_cu(c);
+        const int max_chunks = per_cu > 0 ? per_cu * c->n_cu / nblk : 0;
+        const size_t n_ops = c->ops.size();
+        const size_t n_prog = c->prog_begin.size();
+        c->split_all = build_split_programs(c, max_chunks, true);
+        if (!c->split_all) {                  // no such cut: the tail split
+            c->ops.resize(n_ops);
+            c->prog_begin.resize(n_prog);
+        }
+    }
+    if (!c->split_all) build_split_programs(c, INT_MAX, false);
     return PHYLO_OK;
 }
 
@@ -1526,18 +1607,13 @@ int prepare(phylo_ctx* c) {
     return PHYLO_OK;
 }
 
-// Workgroups per CU below which a whole-tree launch has too few waves to
-// hide its dependent chain: every site block is then walked as subtree
-// pieces plus the top program (the tail split applied to all blocks), which
-// multiplies the independent waves by the number of pieces.
-constexpr int kSplitAllBelowPerCu = 4;
 
 // Site blocks of a whole-tree launch walked as whole-tree workgroups; the
 // rest (the tail) go through the split programs.  Full rounds over the CUs
 // stay whole unless there are too few of them.
 int split_main_blocks(const phylo_ctx* c, int nblk) {
     if (c->n_cu <= 0) return nblk;
-    if (nblk < kSplitAllBelowPerCu * c->n_cu) return 0;
+    if (c->split_all) return 0;
     return nblk / c->n_cu * c->n_cu;
 }
 
@@ -1588,9 +1664,6 @@ int dispatch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
     return PHYLO_OK;
 }
 
-// Site groups per wave of the MFMA walker: two independent MFMA chains per
-// wave share each P operand load and overlap their memory latency.
-constexpr int kMfmaGroups = 2;
 
 template <typename T, int C, int V>
 void launch_mfma(phylo_ctx* c, dim3 grid, double* site_lnl) {
@@ -1675,8 +1748,11 @@ int run_programs(phylo_ctx* c, const Op* d_ops, const int* d_prog, int n_prog, d
     dim3 grid(nblk, n_prog);
     double* dsite = site_lnl_host ? c->d_sitelnl : nullptr;
     if (!c->d_fold) {
+        // cleared on the context's own stream: a null-stream hipMemset is not
+        // ordered with this non-blocking stream, and with other host threads
+        // busy on the device the first launch could find the counter unset
         HIPCHK(c, hipMalloc(&c->d_fold, sizeof(unsigned)));
-        HIPCHK(c, hipMemset(c->d_fold, 0, sizeof(unsigned)));
+        HIPCHK(c, hipMemsetAsync(c->d_fold, 0, sizeof(unsigned), c->stream));
     }
     c->folded = false;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <iostream>
+#include <map>
 #include <random>
 #include <sstream>
 
@@ -21,6 +22,8 @@ simpleML::simpleML() {
     }
     const char* p = std::getenv("PHYLO_AMD_PRECISION");
     if (p) fp64_ = std::string(p) == "fp64";
+    const char* sp = std::getenv("PHYLO_AMD_SITE_PATTERNS");
+    if (sp) patterns_ = std::string(sp) == "1";
 }
 
 simpleML::~simpleML() {
@@ -125,6 +128,28 @@ bool simpleML::prepare(sub_model& model, rate_model& rate_mod, std::vector<int>&
     std::vector<uint32_t> masks((size_t)tip.size() * m);
     for (size_t t = 0; t < tip.size(); ++t)
         for (unsigned int s = 0; s < m; ++s) masks[t * m + s] = (uint32_t)(tipdata_[tip[t]][s].to_ulong() & 0xFFFFFFFFul);
+    // site patterns: the distinct columns, in order of first appearance
+    std::vector<double> weights;
+    unsigned int m_eval = m;
+    site_pattern_.clear();
+    if (patterns_) {
+        std::map<std::vector<uint32_t>, unsigned int> index;
+        std::vector<uint32_t> col(tip.size());
+        site_pattern_.resize(m);
+        for (unsigned int s = 0; s < m; ++s) {
+            for (size_t t = 0; t < tip.size(); ++t) col[t] = masks[t * m + s];
+            auto it = index.emplace(col, (unsigned int)index.size()).first;
+            site_pattern_[s] = it->second;
+            if (it->second == weights.size()) weights.push_back(0.0);
+            weights[it->second] += 1.0;
+        }
+        m_eval = (unsigned int)weights.size();
+        std::vector<uint32_t> packed((size_t)tip.size() * m_eval);
+        for (unsigned int s = 0; s < m; ++s)
+            for (size_t t = 0; t < tip.size(); ++t) packed[t * m_eval + site_pattern_[s]] = masks[t * m + s];
+        masks.swap(packed);
+    }
+    n_patterns_ = m_eval;
     std::vector<int32_t> left(n_lib, -1), right(n_lib, -1);
     for (size_t v = 0; v < nodes_.size(); ++v) {
         if (lib[v] < 0 || is_tip((int)v)) continue;
@@ -140,7 +165,8 @@ bool simpleML::prepare(sub_model& model, rate_model& rate_mod, std::vector<int>&
     int builder = builder_;
     if (builder == P_AUTO) builder = model.is_tr() ? P_EIGEN : P_PADE;
     int rc = phylo_set_precision(ctx_, fp64_ ? PHYLO_PREC_FP64 : PHYLO_PREC_FP32);
-    if (!rc) rc = phylo_set_tips(ctx_, (int)tip.size(), (int)m, (int)S, masks.data());
+    if (!rc) rc = phylo_set_tips(ctx_, (int)tip.size(), (int)m_eval, (int)S, masks.data());
+    if (!rc && patterns_) rc = phylo_set_site_weights(ctx_, weights.data());
     if (!rc) rc = phylo_set_topology(ctx_, n_lib, lib[root_], left.data(), right.data());
     if (!rc) rc = phylo_set_edge_lengths(ctx_, eff.data());
     if (!rc) rc = phylo_set_model(ctx_, (int)S, model.Q().data(), rw.data(), builder == P_EIGEN ? pi.data() : nullptr,
@@ -162,12 +188,18 @@ double simpleML::calculate_log_likelihood(sub_model& model) {
 
 double simpleML::calculate_log_likelihood(sub_model& model, rate_model& rate_mod) {
     if (!prepare(model, rate_mod, lib_of_node_)) return 0.0;
-    site_lnl_.assign(get_n_char(), 0.0);
+    std::vector<double> lnl(n_patterns_, 0.0);
     double total = 0.0;
-    if (phylo_loglik(ctx_, site_lnl_.data(), &total)) {
+    if (phylo_loglik(ctx_, lnl.data(), &total)) {
         err_ = phylo_last_error(ctx_);
         std::cerr << err_ << std::endl;
         return 0.0;
+    }
+    if (site_pattern_.empty()) {
+        site_lnl_.swap(lnl);
+    } else {
+        site_lnl_.resize(site_pattern_.size());
+        for (size_t s = 0; s < site_pattern_.size(); ++s) site_lnl_[s] = lnl[site_pattern_[s]];
     }
     return total;
 }
@@ -195,8 +227,10 @@ void simpleML::draw_normalized_likelihood_on_nodes() {
     if (!ctx_ || lib_of_node_.empty()) return;
     const unsigned int m = get_n_char(), S = n_states_;
     const int C = (int)cat_rates_.size();
-    std::vector<double> clv((size_t)m * C * S);
-    std::vector<int32_t> scale(m);
+    std::vector<double> clv((size_t)n_patterns_ * C * S);
+    std::vector<int32_t> scale(n_patterns_);
+    // the evaluated column of each site (the site itself without patterns)
+    const auto col = [&](unsigned int s) { return site_pattern_.empty() ? s : site_pattern_[s]; };
     // postorder: children first, as the reference's recursion
     std::vector<int> order, stack{root_};
     std::vector<char> seen(nodes_.size(), 0);
@@ -220,10 +254,11 @@ void simpleML::draw_normalized_likelihood_on_nodes() {
             if (s) lab << ",";
             lab << "P={";
             std::vector<double> x(S, 0.0);
-            const double f = std::ldexp(1.0, scale[s]);
+            const unsigned int q = col(s);
+            const double f = std::ldexp(1.0, scale[q]);
             for (unsigned int i = 0; i < S; ++i) {
                 double a = 0.0;
-                for (int c = 0; c < C; ++c) a += (C == 1 ? 1.0 : cat_w_[c]) * clv[((size_t)s * C + c) * S + i];
+                for (int c = 0; c < C; ++c) a += (C == 1 ? 1.0 : cat_w_[c]) * clv[((size_t)q * C + c) * S + i];
                 x[i] = a * f;
                 sum += x[i];
             }

@@ -53,9 +53,10 @@ class simpleML : public tree {
     // The engine options start from the environment, so a program built
     // against these classes unchanged (treeator) can choose them:
     // PHYLO_AMD_P_BUILDER = refcompat | eigen | pade (default: eigen for
-    // reversible models, pade otherwise) and PHYLO_AMD_PRECISION = fp64 |
-    // fp32 (default fp32).  refcompat with fp64 reproduces the reference's
-    // own values to about 1e-11.
+    // reversible models, pade otherwise), PHYLO_AMD_PRECISION = fp64 |
+    // fp32 (default fp32) and PHYLO_AMD_SITE_PATTERNS = 1 (default 0).
+    // refcompat with fp64 reproduces the reference's own values to about
+    // 1e-11.
     simpleML();
     ~simpleML() override;
     simpleML(const simpleML&) = delete;
@@ -86,6 +87,11 @@ class simpleML : public tree {
     // false: the root's child edges keep their lengths (textbook likelihood)
     void set_root_edge_compat(bool on) { root_compat_ = on; }
     void set_simulation_seed(uint64_t seed) { seed_ = seed; seeded_ = true; }
+    // Site-pattern compression (sketched in the reference's MLtree.h:44-47):
+    // identical columns of the tip data are evaluated once, weighted by
+    // their count; per-site results are expanded back to every site.
+    void set_site_patterns(bool on) { patterns_ = on; }
+    unsigned int n_patterns() const { return n_patterns_; }
     const std::vector<double>& site_log_likelihoods() const { return site_lnl_; }
     // Scores the NNI neighbourhood (mode 0: treebender --nni all order,
     // mode 1: every internal edge) of the current tree with the model.
@@ -104,7 +110,9 @@ class simpleML : public tree {
     std::vector<double> cat_rates_{1.0}, cat_w_{1.0};
     std::vector<int> lib_of_node_;
     phylo_ctx* ctx_ = nullptr;
-    bool fp64_ = false, root_compat_ = true, seeded_ = false;
+    std::vector<unsigned int> site_pattern_;       // site -> evaluated column (patterns on)
+    unsigned int n_patterns_ = 0;
+    bool fp64_ = false, root_compat_ = true, seeded_ = false, patterns_ = false;
     int builder_ = P_AUTO;
     unsigned int n_states_ = 0;
     uint64_t seed_ = 0;

@@ -181,13 +181,21 @@ class FitComparison:
     evals_seeded: np.ndarray
     sec_plain: np.ndarray
     sec_seeded: np.ndarray
+    wall_plain: float = 0.0      # wall time of all plain fits (one arm after the other)
+    wall_seeded: float = 0.0
+    workers: int = 1
 
 
 def compare_fits(prop: Proposal, d: ModelDraws, feats, masks, n_cat=4, opts=None, seeded_opts=None,
-                 precision=PREC_FP64) -> FitComparison:
+                 precision=PREC_FP64, workers: int = 1) -> FitComparison:
     """Fits every held-out alignment twice on its true topology (relative
     branch lengths normalised to mean 1): from the default start and from the
-    MLP's prediction.  Returns per-alignment lnL, evaluations and seconds."""
+    MLP's prediction.  Returns per-alignment lnL, evaluations and seconds.
+    With workers > 1 each arm runs its fits on that many host threads (one
+    context, so one HIP stream, per thread): all plain fits, then all seeded
+    ones, and wall_plain / wall_seeded time each arm as a whole."""
+    if workers > 1:
+        return _compare_fits_threaded(prop, d, feats, masks, n_cat, opts, seeded_opts, precision, workers)
     n = len(d.roots)
     out = {k: np.zeros(n) for k in ("lp", "ls", "ep", "es", "tp", "ts")}
     pred = fit_start(prop.predict(feats))
@@ -203,4 +211,42 @@ def compare_fits(prop: Proposal, d: ModelDraws, feats, masks, n_cat=4, opts=None
             _, ls, es, _ = ctx.fit_gtr_gamma(rel, n_cat, pred[k], seeded_opts or opts)
             ts = time.perf_counter() - t0
         out["lp"][k], out["ls"][k], out["ep"][k], out["es"][k], out["tp"][k], out["ts"][k] = lp, ls, ep, es, tp, ts
-    return FitComparison(out["lp"], out["ls"], out["ep"], out["es"], out["tp"], out["ts"])
+    return FitComparison(out["lp"], out["ls"], out["ep"], out["es"], out["tp"], out["ts"],
+                         float(out["tp"].sum()), float(out["ts"].sum()), 1)
+
+
+def _compare_fits_threaded(prop, d, feats, masks, n_cat, opts, seeded_opts, precision, workers):
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+    n = len(d.roots)
+    pred = fit_start(prop.predict(feats))
+    rel = [d.blen[k] / (d.blen[k].sum() / (len(d.blen[k]) - 1)) for k in range(n)]
+    local, made = threading.local(), []
+
+    def fit(k, x0, o):
+        # one context per worker thread, reused by its fits (a context is its
+        # own HIP stream); the library calls release the GIL, so the threads'
+        # fits overlap on the host and their launches on the GPU
+        ctx = getattr(local, "ctx", None)
+        if ctx is None:
+            ctx = local.ctx = Context(0, precision)
+            made.append(ctx)
+        ctx.set_tips(masks[k], 4)
+        ctx.set_topology(d.left[k], d.right[k], int(d.roots[k]))
+        t0 = time.perf_counter()
+        _, lnl, ev, _ = ctx.fit_gtr_gamma(rel[k], n_cat, x0, o)
+        return lnl, ev, time.perf_counter() - t0
+
+    try:
+        with ThreadPoolExecutor(workers) as ex:
+            t0 = time.perf_counter()
+            plain = list(ex.map(lambda k: fit(k, default_start(), opts), range(n)))
+            wall_plain = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            seeded = list(ex.map(lambda k: fit(k, pred[k], seeded_opts or opts), range(n)))
+            wall_seeded = time.perf_counter() - t0
+    finally:
+        for c in made:
+            c.close()
+    P_, S_ = np.array(plain), np.array(seeded)
+    return FitComparison(P_[:, 0], S_[:, 0], P_[:, 1], S_[:, 1], P_[:, 2], S_[:, 2], wall_plain, wall_seeded, workers)

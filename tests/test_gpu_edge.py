@@ -319,3 +319,26 @@ def test_tree_shapes_against_oracle(oracle, shape, keep):
                     want, _ = oracle.Problem(l2, r2, t.root, t.blen, mb, 4, wl.Q, wl.pi, pi=wl.pi,
                                              cat_rates=wl.cat_rates, cat_w=wl.cat_weights).loglik()
                     assert abs(lnl - want) <= 1e-9 * abs(want), (v, side, lnl, want)
+
+
+def test_contexts_on_concurrent_threads():
+    """Fresh contexts evaluated from several host threads at once (as the
+    learned-proposal fits do): every first and later total equals the
+    single-threaded one.  The folded total of a split launch needs the
+    context's counter cleared before its first launch, on its own stream."""
+    import concurrent.futures as cf
+    print("8 threads x 12 fresh contexts, 32 taxa x 2000 sites", flush=True)
+    wl = synth.gtr_gamma_workload(n_tips=32, m=2000, C=4, alpha=0.6, gap_frac=0.05, seed=31)
+    with gpu_context(wl, precision=1) as ctx:
+        want = ctx.loglik()
+
+    def job(k):
+        out = []
+        with gpu_context(wl, precision=1) as c:
+            for _ in range(3):
+                out.append(c.loglik())
+        return out
+
+    with cf.ThreadPoolExecutor(8) as ex:
+        got = list(ex.map(job, range(96)))
+    assert all(v == want for run in got for v in run), {v for run in got for v in run}

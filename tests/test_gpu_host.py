@@ -146,3 +146,30 @@ def test_host_simulate_chars_pattern_frequencies(tmp_path, oracle):
     keep = exp >= 5
     stat = np.sum((counts[keep] - exp[keep]) ** 2 / exp[keep])
     assert stat < chi2.ppf(1 - 1e-6, keep.sum() - 1), stat
+
+
+@pytest.mark.parametrize("name", ["binary_8x40", "dna3_24x300", "t4_2param"])
+def test_site_patterns_give_the_same_values(name):
+    """Site-pattern compression (PHYLO_AMD_SITE_PATTERNS=1): each distinct
+    column evaluated once and weighted by its count gives every site its
+    uncompressed value (FP64 partials: identical) and the same total; the
+    NNI scores see the weights too."""
+    print(f"site patterns {name}", flush=True)
+    args = case_args(name)
+
+    def run(*a, patterns):
+        env = dict(os.environ, PHYLO_AMD_SITE_PATTERNS="1" if patterns else "0")
+        out = subprocess.run([DRIVER, *map(str, a)], capture_output=True, text=True, timeout=300, env=env)
+        assert out.returncode == 0, out.stderr
+        return out.stdout
+
+    site0, tot0 = parse_out(run("persite", *args, "fp64", "refcompat", patterns=False))
+    site1, tot1 = parse_out(run("persite", *args, "fp64", "refcompat", patterns=True))
+    np.testing.assert_array_equal(site1, site0)
+    assert tot1 == pytest.approx(tot0, rel=1e-12, abs=1e-9)
+    want, want_total = read_expected(os.path.join(GOLDEN, name, "expected.txt"))
+    np.testing.assert_allclose(site1, want, rtol=1e-11)
+    if args[2] == "dna":
+        n0 = [float(x.split()[2]) for x in run("nniscore", *args, 1, patterns=False).strip().split("\n")]
+        n1 = [float(x.split()[2]) for x in run("nniscore", *args, 1, patterns=True).strip().split("\n")]
+        np.testing.assert_allclose(n1, n0, rtol=1e-6)
