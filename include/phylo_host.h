@@ -25,8 +25,11 @@ int phylo_gamma_rates(double alpha, int n_cat, double *rates);
  * normalised to mean rate 1 and edge lengths are rel_len[node] * scale.
  * phylo_gtr_gamma_lnl evaluates one point (-inf when invalid);
  * phylo_fit_gtr_gamma maximises from x0 under the Nelder-Mead options nm
- * (NULL: phylo_nm_defaults), restarting from the optimum with a fresh simplex
- * until a restart gains less than ftol_abs, and returns the optimum, its lnL,
+ * (NULL: phylo_nm_defaults).  After the first run it probes each coordinate
+ * reset to its x0 value and reruns the full simplex from the best probe that
+ * gains more than ftol_abs (up to three rounds), then restarts from the
+ * optimum with small simplices until a restart gains less than ftol_abs, and
+ * returns the optimum, its lnL,
  * the number of likelihood evaluations and the wall time. */
 typedef struct phylo_nm_options {
     int max_evals;        /* likelihood evaluations, restarts included */
