@@ -267,16 +267,26 @@ __device__ inline int max_hi_word(const double (&L)[4]) {
 constexpr int kLazyRescaleHi = (1023 - 256) << 20;
 
 // One child's operand for a walker op, by source kind (compile-time).
+// With TL the tip's state set is read from the workgroup's LDS copy and its
+// four 0/1 values from tipv, a 16-entry LDS table (entry 2 code + h: states
+// 2h and 2h + 1 of code): two LDS reads where the bit-by-bit decode takes
+// four bit-field extracts and four conversions.
 template <int K, typename T, int C, bool TL>
-__device__ inline void child4(int idx, const uint8_t* __restrict__ tips, const uint8_t* tl, const T* plane,
-                              size_t slot_stride, const int* scl, int m_pad, int s, int lane, int cat,
+__device__ inline void child4(int idx, const uint8_t* __restrict__ tips, const uint8_t* tl, const double2* tipv,
+                              const T* plane, size_t slot_stride, const int* scl, int m_pad, int s, int lane, int cat,
                               const double (&cA)[4], int sA, double2 (&xb)[C][2][64], int sB, double (&x)[4], int& sc) {
     if (K == SRC_TIP) {
-        const unsigned code = TL ? (tl[idx * 32 + (lane >> 1)] >> ((lane & 1) * 4)) & 15u : tips[(size_t)idx * m_pad + s];
-        x[0] = tip_bit<0>(code);
-        x[1] = tip_bit<1>(code);
-        x[2] = tip_bit<2>(code);
-        x[3] = tip_bit<3>(code);
+        if (TL) {
+            const unsigned code = (tl[idx * 32 + (lane >> 1)] >> ((lane & 1) * 4)) & 15u;
+            const double2 a = tipv[2 * code], b = tipv[2 * code + 1];
+            x[0] = a.x; x[1] = a.y; x[2] = b.x; x[3] = b.y;
+        } else {
+            const unsigned code = tips[(size_t)idx * m_pad + s];
+            x[0] = tip_bit<0>(code);
+            x[1] = tip_bit<1>(code);
+            x[2] = tip_bit<2>(code);
+            x[3] = tip_bit<3>(code);
+        }
     } else if (K == SRC_CLV) {
         load4(plane + idx * slot_stride, x);
         sc += scl[((size_t)idx * C + cat) * m_pad + s];
@@ -296,12 +306,14 @@ __device__ inline void child4(int idx, const uint8_t* __restrict__ tips, const u
 // one straight-line body each; testing the kinds one by one costs the
 // compiler's flag-and-branch glue on every op (about 10 % of the C2 kernel).
 template <int LK, int RK, typename T, int C, bool TL>
-__device__ inline void children4(const Op& op, const uint8_t* __restrict__ tips, const uint8_t* tl, const T* plane,
+__device__ inline void children4(const Op& op, const uint8_t* __restrict__ tips, const uint8_t* tl,
+                                 const double2* tipv, const T* plane,
                                  size_t slot_stride, const int* scl, int m_pad, int s, int lane, int cat,
                                  const double (&cA)[4], int sA, double2 (&xb)[C][2][64], int sB,
                                  const double (&Pl)[16], const double (&Pr)[16], double (&L)[4], int& sc) {
     double x[4];
-    child4<LK, T, C, TL>(op.l_idx, tips, tl, plane, slot_stride, scl, m_pad, s, lane, cat, cA, sA, xb, sB, x, sc);
+    child4<LK, T, C, TL>(op.l_idx, tips, tl, tipv, plane, slot_stride, scl, m_pad, s, lane, cat, cA, sA, xb, sB, x,
+                         sc);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         double a = 0.0;
@@ -310,8 +322,8 @@ __device__ inline void children4(const Op& op, const uint8_t* __restrict__ tips,
         L[i] = a;
     }
     if (RK != SRC_NONE) {
-        child4<RK, T, C, TL>(op.r_idx, tips, tl, plane, slot_stride, scl, m_pad, s, lane, cat, cA, sA, xb, sB, x,
-                             sc);
+        child4<RK, T, C, TL>(op.r_idx, tips, tl, tipv, plane, slot_stride, scl, m_pad, s, lane, cat, cA, sA, xb, sB,
+                             x, sc);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             double a = 0.0;
@@ -333,14 +345,17 @@ __device__ inline void children4(const Op& op, const uint8_t* __restrict__ tips,
 // A stored partial is scaled so its largest entry is in [0.5, 1) (the
 // scaler is the frexp exponent of its maximum); a carried one is rescaled
 // only when it drops below 2^-256.
-// walk4 runs ops[k0, k1) for site s and category cat; at the root op it sets
+// walk4 runs ops[k0, k1) for site s and category cat; when the last op is
+// the root (a program's root op is always its last) it sets
 // root_lc = w_cat sum_i rw_i L_i and root_sc, the category's scaler, so that
-// its share of the site likelihood is root_lc 2^root_sc.
+// its share of the site likelihood is root_lc 2^root_sc (the root is read
+// from carry A after the loop, so the loop carries no root values).
 // With TL the tip codes of the workgroup's 64 sites come from LDS (tl: taxon
-// t's sites 2b and 2b+1 in the low and high nibble of tl[32 t + b]).
+// t's sites 2b and 2b+1 in the low and high nibble of tl[32 t + b]) and their
+// 0/1 state values from the LDS table tipv (child4).
 template <typename T, int C, bool TL>
 __device__ inline void walk4(const Op* __restrict__ ops, int k0, int k1, const uint8_t* __restrict__ tips,
-                             const uint8_t* tl, T* clv, int* scl, const double* __restrict__ P,
+                             const uint8_t* tl, const double2* tipv, T* clv, int* scl, const double* __restrict__ P,
                              const double* __restrict__ roots, int m_pad, int s, int lane, int cat,
                              double2 (&xb)[C][2][64], double& root_lc, int& root_sc) {
     // carried results stay in FP64: only stored partials are rounded to T
@@ -348,17 +363,18 @@ __device__ inline void walk4(const Op* __restrict__ ops, int k0, int k1, const u
     // sits in LDS: xb[cat][][lane], written and read by the same lane)
     double cA[4] = {0.0, 0.0, 0.0, 0.0};
     int sA = 0, sB = 0;
-    root_lc = 0.0;
-    root_sc = 0;
     const T* plane = clv + ((size_t)cat * m_pad + s) * 4;          // + slot * C * m_pad * 4
     const size_t slot_stride = (size_t)C * m_pad * 4;
+    const char* Pcat = reinterpret_cast<const char*>(P + (size_t)cat * 16);
     for (int k = k0; k < k1; ++k) {
         const Op op = ops[k];
         // both children's P up front (64 SGPRs): their scalar loads overlap
+        // (32-bit byte offsets from this category's first matrix: three
+        // scalar instructions per address instead of five)
         double Pl[16], Pr[16];
         {
-            const double* pl = P + ((size_t)op.l_node * C + cat) * 16;
-            const double* pr = P + ((size_t)op.r_node * C + cat) * 16;
+            const double* pl = reinterpret_cast<const double*>(Pcat + (uint32_t)op.l_node * (uint32_t)(C * 128));
+            const double* pr = reinterpret_cast<const double*>(Pcat + (uint32_t)op.r_node * (uint32_t)(C * 128));
 #pragma unroll
             for (int i = 0; i < 16; ++i) { Pl[i] = pl[i]; Pr[i] = pr[i]; }
         }
@@ -368,8 +384,8 @@ __device__ inline void walk4(const Op* __restrict__ ops, int k0, int k1, const u
         switch (op.l_kind * 5 + op.r_kind) {
 #define PHYLO_SHAPE(A, B)                                                                                   \
     case A * 5 + B:                                                                                       \
-        children4<A, B, T, C, TL>(op, tips, tl, plane, slot_stride, scl, m_pad, s, lane, cat, cA, sA, xb, sB, Pl, \
-                                  Pr, L, sc);                                                             \
+        children4<A, B, T, C, TL>(op, tips, tl, tipv, plane, slot_stride, scl, m_pad, s, lane, cat, cA, sA, xb, sB, \
+                                  Pl, Pr, L, sc);                                                         \
         break;
             PHYLO_SHAPE(0, 0) PHYLO_SHAPE(0, 1) PHYLO_SHAPE(0, 2) PHYLO_SHAPE(0, 3) PHYLO_SHAPE(0, 4)
             PHYLO_SHAPE(1, 0) PHYLO_SHAPE(1, 1) PHYLO_SHAPE(1, 2) PHYLO_SHAPE(1, 3) PHYLO_SHAPE(1, 4)
@@ -409,23 +425,26 @@ __device__ inline void walk4(const Op* __restrict__ ops, int k0, int k1, const u
 #pragma unroll
         for (int i = 0; i < 4; ++i) cA[i] = L[i];
         sA = sc;
-        if (op.flags & OP_TO_B) {
+        const bool to_b = op.flags & OP_TO_B;
+        if (to_b) {
             xb[cat][0][lane] = make_double2(L[0], L[1]);
             xb[cat][1][lane] = make_double2(L[2], L[3]);
-            sB = sc;
         }
-        if (op.flags & OP_ROOT) {
-            // the root partial is brought to [0.5, 1) first, so the site
-            // value does not depend on where the walk rescaled (a program
-            // that reads a stored child and one that carries it reach the
-            // root with the same mantissas)
-            const int e = max_exponent(L);
-            double lc = 0.0;
+        sB = to_b ? sc : sB;
+    }
+    root_lc = 0.0;
+    root_sc = 0;
+    if (k1 > k0 && (ops[k1 - 1].flags & OP_ROOT)) {
+        // the root partial (carry A) is brought to [0.5, 1) first, so the
+        // site value does not depend on where the walk rescaled (a program
+        // that reads a stored child and one that carries it reach the root
+        // with the same mantissas)
+        const int e = max_exponent(cA);
+        double lc = 0.0;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) lc = fma(roots[i], ldexp(L[i], -e), lc);
-            root_lc = lc * roots[kRootCatOff + cat];
-            root_sc = sc + e;
-        }
+        for (int i = 0; i < 4; ++i) lc = fma(roots[i], ldexp(cA[i], -e), lc);
+        root_lc = lc * roots[kRootCatOff + cat];
+        root_sc = sA + e;
     }
 }
 
@@ -476,6 +495,7 @@ __global__ __launch_bounds__(64 * C) void prune4_kernel(
     __shared__ double xroot[C][64];
     __shared__ double2 xb[C][2][64];
     __shared__ uint8_t tl[TL ? kTipLdsTaxa * 32 : 1];
+    __shared__ double2 tipv[TL ? 32 : 1];
     const int lane = threadIdx.x & 63;
     const int cat = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int y = blockIdx.y;
@@ -494,6 +514,10 @@ __global__ __launch_bounds__(64 * C) void prune4_kernel(
     }
     const int s = bx * 64 + lane;
     if (TL) {
+        if (threadIdx.x < 32) {          // 0/1 values of states 2h, 2h + 1 of each code
+            const unsigned code = threadIdx.x >> 1, h = (threadIdx.x & 1) * 2;
+            tipv[threadIdx.x] = make_double2((double)((code >> h) & 1u), (double)((code >> (h + 1)) & 1u));
+        }
         const uint8_t* blk = tips + (size_t)bx * 64;
         for (int i = threadIdx.x; i < n_taxa * 32; i += 64 * C) {
             const uint32_t pr = *reinterpret_cast<const uint16_t*>(blk + (size_t)(i >> 5) * m_pad + 2 * (i & 31));
@@ -503,8 +527,8 @@ __global__ __launch_bounds__(64 * C) void prune4_kernel(
     }
     double lc;
     int sc;
-    walk4<T, C, TL>(ops, prog_begin[prog], prog_begin[prog + 1], tips, tl, clv, scl, P, roots, m_pad, s, lane, cat, xb,
-                    lc, sc);
+    walk4<T, C, TL>(ops, prog_begin[prog], prog_begin[prog + 1], tips, tl, tipv, clv, scl, P, roots, m_pad, s, lane,
+                    cat, xb, lc, sc);
     if (!has_root) return;    // a piece: its stores are its result (uniform per workgroup)
     const double lnl = combine_categories<C>(lc, sc, cat, lane, xroot, xsc);
     const bool live = s < m && cat == 0;
@@ -566,16 +590,17 @@ __global__ __launch_bounds__(64 * CW) void batch4_kernel(const WorkItem* __restr
             if (CW == C) {
                 double lc;
                 int sc;
-                walk4<T, C, false>(ops, k0, k1, tips + I.tips_off, nullptr, clv + I.clv_off, scl + I.scl_off,
-                                   P + I.P_off, roots + I.roots_off, I.m_pad, s, lane, wcat, xb, lc, sc);
+                walk4<T, C, false>(ops, k0, k1, tips + I.tips_off, nullptr, nullptr, clv + I.clv_off,
+                                   scl + I.scl_off, P + I.P_off, roots + I.roots_off, I.m_pad, s, lane, wcat, xb, lc,
+                                   sc);
                 lnl = combine_categories<C>(lc, sc, wcat, lane, xroot, xsc);
             } else {
                 double lc[C];
                 int sc[C];
                 for (int cat = 0; cat < C; ++cat)
-                    walk4<T, C, false>(ops, k0, k1, tips + I.tips_off, nullptr, clv + I.clv_off, scl + I.scl_off,
-                                       P + I.P_off, roots + I.roots_off, I.m_pad, s, lane, cat, xb, lc[cat],
-                                       sc[cat]);
+                    walk4<T, C, false>(ops, k0, k1, tips + I.tips_off, nullptr, nullptr, clv + I.clv_off,
+                                       scl + I.scl_off, P + I.P_off, roots + I.roots_off, I.m_pad, s, lane, cat, xb,
+                                       lc[cat], sc[cat]);
                 int M = sc[0];
 #pragma unroll
                 for (int c = 1; c < C; ++c) M = max(M, sc[c]);
@@ -1266,6 +1291,17 @@ bool build_full_ops(const TV* c, std::vector<Op>& ops) {
     return true;
 }
 
+// The DNA walker reads a program's root value from carry A after its last
+// op, so a root op must end its program.  Every program builder puts the root
+// last (children before parents); this checks it before programs go to the
+// device.
+bool root_ops_last(const std::vector<Op>& ops, const std::vector<int>& begin) {
+    for (size_t p = 0; p + 1 < begin.size(); ++p)
+        for (int k = begin[p]; k < begin[p + 1]; ++k)
+            if ((ops[k].flags & OP_ROOT) && k != begin[p + 1] - 1) return false;
+    return true;
+}
+
 // Clears the stores of ops[begin, end) whose result no op of that range reads
 // back from memory: results consumed only through the two carries.  With the
 // second carry only results that wait while a branching subtree is walked keep
@@ -1576,6 +1612,7 @@ int prepare(phylo_ctx* c) {
     if (c->dirty_topo) {
         int rc = compile_full_program(c);
         if (rc) return rc;
+        if (!root_ops_last(c->ops, c->prog_begin)) return fail(c, PHYLO_ERR_STATE, "internal: root op not last");
         HIPCHK(c, ensure(c->d_ops, c->ops_bytes, c->ops.size() * sizeof(Op)));
         HIPCHK(c, ensure(c->d_prog, c->prog_bytes, c->prog_begin.size() * sizeof(int)));
         HIPCHK(c, hipMemcpyAsync(c->d_ops, c->ops.data(), c->ops.size() * sizeof(Op), hipMemcpyHostToDevice, c->stream));
@@ -2261,6 +2298,7 @@ int phylo_nni_score(phylo_ctx* c, int mode, int max_variants, double* lnl, int32
     }
     const int n_prog = (int)which.size();
     if (n_prog == 0) return PHYLO_OK;
+    if (!root_ops_last(ops, begin)) return fail(c, PHYLO_ERR_STATE, "internal: root op not last");
     HIPCHK(c, ensure(c->d_nni_ops, c->nni_ops_bytes, ops.size() * sizeof(Op)));
     HIPCHK(c, ensure(c->d_nni_prog, c->nni_prog_bytes, begin.size() * sizeof(int)));
     HIPCHK(c, hipMemcpyAsync(c->d_nni_ops, ops.data(), ops.size() * sizeof(Op), hipMemcpyHostToDevice, c->stream));
@@ -2456,6 +2494,7 @@ extern "C" int phylo_loglik_noalloc(phylo_ctx* c, int n_taxa, int n_sites, int n
     tv.right.assign(right, right + n_nodes);
     std::vector<Op> ops;
     if (!build_full_ops(&tv, ops)) return fail(c, PHYLO_ERR_ARG, "loglik_noalloc: internal nodes unreachable");
+    if (!root_ops_last(ops, {0, (int)ops.size()})) return fail(c, PHYLO_ERR_STATE, "internal: root op not last");
     drop_unread_stores(ops, 0, ops.size());
     std::vector<int> dense(n_nodes - n_taxa, -1);
     int n_store = 0;
@@ -2919,6 +2958,7 @@ int phylo_batch_finalize(phylo_batch* b, int autotune) {
         b->var_offsets.push_back(b->var_offsets.back() + (int)I.nni_node.size());
     }
     b->n_var = b->var_offsets.back();
+    if (!root_ops_last(ops, prog_begin)) return fail(b, PHYLO_ERR_STATE, "internal: root op not last");
     // device buffers
     const size_t elt = b->prec == PHYLO_PREC_FP64 ? sizeof(double) : sizeof(float);
     HIPCHK(b, hipMalloc(&b->d_tips, std::max<long>(tips_bytes, 1)));

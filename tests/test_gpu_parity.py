@@ -47,6 +47,10 @@ WIDE_CASES = [
     (12, 900, 23, 4, 0.5, 0.05, 14),
     (10, 700, 32, 8, 1.2, 0.02, 15),
     (20, 1200, 13, 4, 0.6, 0.05, 16),
+    # more taxa than the matrix-core walker stages in LDS (128): tips read
+    # from global memory on the whole-tree path
+    (160, 1500, 20, 4, 1.0, 0.03, 17),
+    (200, 700, 23, 2, 0.7, 0.05, 18),
 ]
 
 
