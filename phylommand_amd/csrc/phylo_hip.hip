@@ -353,6 +353,81 @@ __device__ inline void children4(const Op& op, const uint8_t* __restrict__ tips,
 // With TL the tip codes of the workgroup's 64 sites come from LDS (tl: taxon
 // t's sites 2b and 2b+1 in the low and high nibble of tl[32 t + b]) and their
 // 0/1 state values from the LDS table tipv (child4).
+// One op of walk4 (ops[k] for site s and category cat).  Carry A comes in
+// as cin / sin and the result leaves as cout / sout: walk4 alternates two
+// register sets, so the result is written where the next op reads it, with
+// no copy.
+template <typename T, int C, bool TL>
+__device__ __forceinline__ void walk4_op(const Op op, const uint8_t* __restrict__ tips, const uint8_t* tl,
+                                         const double2* tipv, const T* plane, size_t slot_stride, int* scl,
+                                         const char* Pcat, int m_pad, int s, int lane, int cat,
+                                         double2 (&xb)[C][2][64], const double (&cin)[4], int sin, int& sB,
+                                         double (&cout)[4], int& sout) {
+    // both children's P up front (64 SGPRs): their scalar loads overlap
+    // (32-bit byte offsets from this category's first matrix: three
+    // scalar instructions per address instead of five)
+    double Pl[16], Pr[16];
+    {
+        const double* pl = reinterpret_cast<const double*>(Pcat + (uint32_t)op.l_node * (uint32_t)(C * 128));
+        const double* pr = reinterpret_cast<const double*>(Pcat + (uint32_t)op.r_node * (uint32_t)(C * 128));
+#pragma unroll
+        for (int i = 0; i < 16; ++i) { Pl[i] = pl[i]; Pr[i] = pr[i]; }
+    }
+    // child inputs: one straight-line body per (left kind, right kind)
+    double L[4];
+    int sc = 0;
+    switch (op.l_kind * 5 + op.r_kind) {
+#define PHYLO_SHAPE(A, B)                                                                                   \
+    case A * 5 + B:                                                                                       \
+        children4<A, B, T, C, TL>(op, tips, tl, tipv, plane, slot_stride, scl, m_pad, s, lane, cat, cin, sin, xb, \
+                                  sB, Pl, Pr, L, sc);                                                     \
+        break;
+        PHYLO_SHAPE(0, 0) PHYLO_SHAPE(0, 1) PHYLO_SHAPE(0, 2) PHYLO_SHAPE(0, 3) PHYLO_SHAPE(0, 4)
+        PHYLO_SHAPE(1, 0) PHYLO_SHAPE(1, 1) PHYLO_SHAPE(1, 2) PHYLO_SHAPE(1, 3) PHYLO_SHAPE(1, 4)
+        PHYLO_SHAPE(2, 0) PHYLO_SHAPE(2, 1) PHYLO_SHAPE(2, 2) PHYLO_SHAPE(2, 3) PHYLO_SHAPE(2, 4)
+        default: PHYLO_SHAPE(3, 0) PHYLO_SHAPE(3, 1) PHYLO_SHAPE(3, 2) PHYLO_SHAPE(3, 3) PHYLO_SHAPE(3, 4)
+#undef PHYLO_SHAPE
+    }
+    // Carried results are FP64 and rescaled lazily: only when the
+    // largest entry falls below 2^-256 (rare; a lane-divergent branch
+    // costs the others one compare), so an op's children are each at
+    // least 2^-256 and their product stays far above the FP64 normal
+    // range.  (Rescaling by a power of two is exact, so where it
+    // happens changes no value.)
+    if (max_hi_word(L) < kLazyRescaleHi) {
+        const int e = max_exponent(L);
+        sc += e;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) L[i] = ldexp(L[i], -e);
+    }
+    if (op.out_slot >= 0) {
+        // a stored partial is brought to [0.5, 1) with its own scaler
+        // (the carried value keeps its scale, so programs that store
+        // and programs that carry a node compute the same values), then
+        // rounded to T: most results are only carried (the program
+        // stores what it reads back), so the rounding sits here, on the
+        // few stored ones
+        const int e = max_exponent(L);
+        const uint32_t key = round_key(s, op.l_node, op.r_node, cat);
+        T r[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) r[i] = (T)round_cut(ldexp(L[i], -e), key + 0x9e3779b9u * i, T());
+        store4(const_cast<T*>(plane) + op.out_slot * slot_stride, r);
+        scl[((size_t)op.out_slot * C + cat) * m_pad + s] = sc + e;
+    }
+    // carry A takes every result (after an op to carry B nothing reads
+    // the old A), so no select between the carries
+#pragma unroll
+    for (int i = 0; i < 4; ++i) cout[i] = L[i];
+    sout = sc;
+    const bool to_b = op.flags & OP_TO_B;
+    if (to_b) {
+        xb[cat][0][lane] = make_double2(L[0], L[1]);
+        xb[cat][1][lane] = make_double2(L[2], L[3]);
+    }
+    sB = to_b ? sc : sB;
+}
+
 template <typename T, int C, bool TL>
 __device__ inline void walk4(const Op* __restrict__ ops, int k0, int k1, const uint8_t* __restrict__ tips,
                              const uint8_t* tl, const double2* tipv, T* clv, int* scl, const double* __restrict__ P,
@@ -366,71 +441,22 @@ __device__ inline void walk4(const Op* __restrict__ ops, int k0, int k1, const u
     const T* plane = clv + ((size_t)cat * m_pad + s) * 4;          // + slot * C * m_pad * 4
     const size_t slot_stride = (size_t)C * m_pad * 4;
     const char* Pcat = reinterpret_cast<const char*>(P + (size_t)cat * 16);
-    for (int k = k0; k < k1; ++k) {
-        const Op op = ops[k];
-        // both children's P up front (64 SGPRs): their scalar loads overlap
-        // (32-bit byte offsets from this category's first matrix: three
-        // scalar instructions per address instead of five)
-        double Pl[16], Pr[16];
-        {
-            const double* pl = reinterpret_cast<const double*>(Pcat + (uint32_t)op.l_node * (uint32_t)(C * 128));
-            const double* pr = reinterpret_cast<const double*>(Pcat + (uint32_t)op.r_node * (uint32_t)(C * 128));
+    double cB2[4] = {0.0, 0.0, 0.0, 0.0};
+    int sA2 = 0;
+    // two ops per turn, carry A alternating between cA and cB2
+    int k = k0;
+    for (; k + 1 < k1; k += 2) {
+        walk4_op<T, C, TL>(ops[k], tips, tl, tipv, plane, slot_stride, scl, Pcat, m_pad, s, lane, cat, xb, cA, sA, sB,
+                           cB2, sA2);
+        walk4_op<T, C, TL>(ops[k + 1], tips, tl, tipv, plane, slot_stride, scl, Pcat, m_pad, s, lane, cat, xb, cB2,
+                           sA2, sB, cA, sA);
+    }
+    if (k < k1) {       // an odd op count: the last op, its result back in cA
+        walk4_op<T, C, TL>(ops[k], tips, tl, tipv, plane, slot_stride, scl, Pcat, m_pad, s, lane, cat, xb, cA, sA, sB,
+                           cB2, sA2);
 #pragma unroll
-            for (int i = 0; i < 16; ++i) { Pl[i] = pl[i]; Pr[i] = pr[i]; }
-        }
-        // child inputs: one straight-line body per (left kind, right kind)
-        double L[4];
-        int sc = 0;
-        switch (op.l_kind * 5 + op.r_kind) {
-#define PHYLO_SHAPE(A, B)                                                                                   \
-    case A * 5 + B:                                                                                       \
-        children4<A, B, T, C, TL>(op, tips, tl, tipv, plane, slot_stride, scl, m_pad, s, lane, cat, cA, sA, xb, sB, \
-                                  Pl, Pr, L, sc);                                                         \
-        break;
-            PHYLO_SHAPE(0, 0) PHYLO_SHAPE(0, 1) PHYLO_SHAPE(0, 2) PHYLO_SHAPE(0, 3) PHYLO_SHAPE(0, 4)
-            PHYLO_SHAPE(1, 0) PHYLO_SHAPE(1, 1) PHYLO_SHAPE(1, 2) PHYLO_SHAPE(1, 3) PHYLO_SHAPE(1, 4)
-            PHYLO_SHAPE(2, 0) PHYLO_SHAPE(2, 1) PHYLO_SHAPE(2, 2) PHYLO_SHAPE(2, 3) PHYLO_SHAPE(2, 4)
-            default: PHYLO_SHAPE(3, 0) PHYLO_SHAPE(3, 1) PHYLO_SHAPE(3, 2) PHYLO_SHAPE(3, 3) PHYLO_SHAPE(3, 4)
-#undef PHYLO_SHAPE
-        }
-        // Carried results are FP64 and rescaled lazily: only when the
-        // largest entry falls below 2^-256 (rare; a lane-divergent branch
-        // costs the others one compare), so an op's children are each at
-        // least 2^-256 and their product stays far above the FP64 normal
-        // range.  (Rescaling by a power of two is exact, so where it
-        // happens changes no value.)
-        if (max_hi_word(L) < kLazyRescaleHi) {
-            const int e = max_exponent(L);
-            sc += e;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) L[i] = ldexp(L[i], -e);
-        }
-        if (op.out_slot >= 0) {
-            // a stored partial is brought to [0.5, 1) with its own scaler
-            // (the carried value keeps its scale, so programs that store
-            // and programs that carry a node compute the same values), then
-            // rounded to T: most results are only carried (the program
-            // stores what it reads back), so the rounding sits here, on the
-            // few stored ones
-            const int e = max_exponent(L);
-            const uint32_t key = round_key(s, op.l_node, op.r_node, cat);
-            T r[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) r[i] = (T)round_cut(ldexp(L[i], -e), key + 0x9e3779b9u * i, T());
-            store4(const_cast<T*>(plane) + op.out_slot * slot_stride, r);
-            scl[((size_t)op.out_slot * C + cat) * m_pad + s] = sc + e;
-        }
-        // carry A takes every result (after an op to carry B nothing reads
-        // the old A), so no select between the carries
-#pragma unroll
-        for (int i = 0; i < 4; ++i) cA[i] = L[i];
-        sA = sc;
-        const bool to_b = op.flags & OP_TO_B;
-        if (to_b) {
-            xb[cat][0][lane] = make_double2(L[0], L[1]);
-            xb[cat][1][lane] = make_double2(L[2], L[3]);
-        }
-        sB = to_b ? sc : sB;
+        for (int i = 0; i < 4; ++i) cA[i] = cB2[i];
+        sA = sA2;
     }
     root_lc = 0.0;
     root_sc = 0;
