@@ -428,7 +428,11 @@ __device__ __forceinline__ void walk4_op(const Op op, const uint8_t* __restrict_
     sB = to_b ? sc : sB;
 }
 
-template <typename T, int C, bool TL>
+// PAIRS: two ops per loop turn (the whole-tree and NNI launches); without
+// it one op per turn, the result copied into carry A (the batch kernel,
+// where the second copy of the op body would cost it a resident wave).  The
+// values are the same either way.
+template <typename T, int C, bool TL, bool PAIRS = true>
 __device__ inline void walk4(const Op* __restrict__ ops, int k0, int k1, const uint8_t* __restrict__ tips,
                              const uint8_t* tl, const double2* tipv, T* clv, int* scl, const double* __restrict__ P,
                              const double* __restrict__ roots, int m_pad, int s, int lane, int cat,
@@ -443,20 +447,30 @@ __device__ inline void walk4(const Op* __restrict__ ops, int k0, int k1, const u
     const char* Pcat = reinterpret_cast<const char*>(P + (size_t)cat * 16);
     double cB2[4] = {0.0, 0.0, 0.0, 0.0};
     int sA2 = 0;
-    // two ops per turn, carry A alternating between cA and cB2
     int k = k0;
-    for (; k + 1 < k1; k += 2) {
-        walk4_op<T, C, TL>(ops[k], tips, tl, tipv, plane, slot_stride, scl, Pcat, m_pad, s, lane, cat, xb, cA, sA, sB,
-                           cB2, sA2);
-        walk4_op<T, C, TL>(ops[k + 1], tips, tl, tipv, plane, slot_stride, scl, Pcat, m_pad, s, lane, cat, xb, cB2,
-                           sA2, sB, cA, sA);
-    }
-    if (k < k1) {       // an odd op count: the last op, its result back in cA
-        walk4_op<T, C, TL>(ops[k], tips, tl, tipv, plane, slot_stride, scl, Pcat, m_pad, s, lane, cat, xb, cA, sA, sB,
-                           cB2, sA2);
+    if constexpr (!PAIRS) {
+        for (; k < k1; ++k) {
+            walk4_op<T, C, TL>(ops[k], tips, tl, tipv, plane, slot_stride, scl, Pcat, m_pad, s, lane, cat, xb, cA, sA,
+                               sB, cB2, sA2);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) cA[i] = cB2[i];
-        sA = sA2;
+            for (int i = 0; i < 4; ++i) cA[i] = cB2[i];
+            sA = sA2;
+        }
+    } else {
+        // two ops per turn, carry A alternating between cA and cB2
+        for (; k + 1 < k1; k += 2) {
+            walk4_op<T, C, TL>(ops[k], tips, tl, tipv, plane, slot_stride, scl, Pcat, m_pad, s, lane, cat, xb, cA, sA,
+                               sB, cB2, sA2);
+            walk4_op<T, C, TL>(ops[k + 1], tips, tl, tipv, plane, slot_stride, scl, Pcat, m_pad, s, lane, cat, xb,
+                               cB2, sA2, sB, cA, sA);
+        }
+        if (k < k1) {       // an odd op count: the last op, its result back in cA
+            walk4_op<T, C, TL>(ops[k], tips, tl, tipv, plane, slot_stride, scl, Pcat, m_pad, s, lane, cat, xb, cA, sA,
+                               sB, cB2, sA2);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) cA[i] = cB2[i];
+            sA = sA2;
+        }
     }
     root_lc = 0.0;
     root_sc = 0;
@@ -616,17 +630,17 @@ __global__ __launch_bounds__(64 * CW) void batch4_kernel(const WorkItem* __restr
             if (CW == C) {
                 double lc;
                 int sc;
-                walk4<T, C, false>(ops, k0, k1, tips + I.tips_off, nullptr, nullptr, clv + I.clv_off,
-                                   scl + I.scl_off, P + I.P_off, roots + I.roots_off, I.m_pad, s, lane, wcat, xb, lc,
-                                   sc);
+                walk4<T, C, false, false>(ops, k0, k1, tips + I.tips_off, nullptr, nullptr, clv + I.clv_off,
+                                          scl + I.scl_off, P + I.P_off, roots + I.roots_off, I.m_pad, s, lane, wcat,
+                                          xb, lc, sc);
                 lnl = combine_categories<C>(lc, sc, wcat, lane, xroot, xsc);
             } else {
                 double lc[C];
                 int sc[C];
                 for (int cat = 0; cat < C; ++cat)
-                    walk4<T, C, false>(ops, k0, k1, tips + I.tips_off, nullptr, nullptr, clv + I.clv_off,
-                                       scl + I.scl_off, P + I.P_off, roots + I.roots_off, I.m_pad, s, lane, cat, xb,
-                                       lc[cat], sc[cat]);
+                    walk4<T, C, false, false>(ops, k0, k1, tips + I.tips_off, nullptr, nullptr,
+                                              clv + I.clv_off, scl + I.scl_off, P + I.P_off, roots + I.roots_off,
+                                              I.m_pad, s, lane, cat, xb, lc[cat], sc[cat]);
                 int M = sc[0];
 #pragma unroll
                 for (int c = 1; c < C; ++c) M = max(M, sc[c]);
