@@ -742,6 +742,26 @@ __global__ void ptm_kernel(const double* __restrict__ P, int S, int Sp, int C, i
 __device__ inline double to_d(float x) { return (double)x; }
 __device__ inline double to_d(double x) { return x; }
 
+// Largest state value of a site in the matrix walker's layout (states 4t + g
+// in slot t of lane group g, so the maximum runs over the slots and then the
+// four lane groups: lanes l ^ 16, l ^ 32), and its frexp exponent (0 when no
+// state is positive).  Every lane of the wave must take part.
+template <int V>
+__device__ inline double site_max(const double (&v)[V]) {
+    double mx = 0.0;
+#pragma unroll
+    for (int t = 0; t < V; ++t) mx = fmax(mx, v[t]);
+    mx = fmax(mx, __shfl_xor(mx, 16, 64));
+    return fmax(mx, __shfl_xor(mx, 32, 64));
+}
+template <int V>
+__device__ inline int site_exponent(const double (&v)[V]) {
+    const double mx = site_max<V>(v);
+    int e = 0;
+    if (mx > 0.0) (void)frexp(mx, &e);
+    return e;
+}
+
 // One child for the G site groups of a wave: acc[q][T] = P . x for tile T,
 // x in operand order.  The P operand of each K-step is loaded once and used by
 // every group.
@@ -751,8 +771,8 @@ __device__ inline void mfma_child(int kind, int idx, int node, int cat, const in
                                   const uint32_t* __restrict__ tips, const uint32_t* tl, const T* clv,
                                   size_t slot_stride,
                                   const int* scl, const double* __restrict__ Pm, const double* __restrict__ PTm,
-                                  const T (&cA)[G][V], const int (&sA)[G], const T (&cB)[G][V], const int (&sB)[G],
-                                  mfma_d4 (&acc)[G][2], int (&sc)[G]) {
+                                  const double (&cA)[G][V], const int (&sA)[G], const double (&cB)[G][V],
+                                  const int (&sB)[G], mfma_d4 (&acc)[G][2], int (&sc)[G]) {
     constexpr int TN = V > 4 ? 2 : 1;
     double x[G][V];
     if (kind == SRC_TIP) {
@@ -842,7 +862,7 @@ constexpr int kMfmaTipLdsTaxa = 128;
 // TL (n_taxa <= kMfmaTipLdsTaxa): the workgroup first copies the tip masks of
 // its 16 G sites into LDS (tl[16 G t + site]).
 template <typename T, int C, int V, int G, bool TL>
-__global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
+__global__ __launch_bounds__(64 * C, (V <= 5 ? 3 : 1)) void prune_mfma_kernel(
     const Op* __restrict__ ops, const int* __restrict__ prog_begin, const uint32_t* __restrict__ tips, T* clv,
     int* scl, const double* __restrict__ Pm, const double* __restrict__ PTm, const double* __restrict__ roots,
     const double* __restrict__ site_w, double* __restrict__ site_lnl, double* __restrict__ partial, int m, int m_pad,
@@ -881,14 +901,14 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
     for (int q = 0; q < G; ++q) s[q] = (sg0 + q) * 16 + j;
     const int k0 = prog_begin[prog], k1 = prog_begin[prog + 1];
     const size_t slot_stride = (size_t)C * m_pad * 4 * V;
-    // carried results are rounded to T like stored ones (registers are the
-    // limit of this walker's occupancy)
-    T cA[G][V], cB[G][V];
+    // carried results stay in FP64 and are rescaled lazily, as in the DNA
+    // walker; only stored partials are rounded to T
+    double cA[G][V], cB[G][V];
     int sA[G], sB[G];
 #pragma unroll
     for (int q = 0; q < G; ++q) {
 #pragma unroll
-        for (int t = 0; t < V; ++t) { cA[q][t] = T(0); cB[q][t] = T(0); }
+        for (int t = 0; t < V; ++t) { cA[q][t] = 0.0; cB[q][t] = 0.0; }
         sA[q] = 0;
         sB[q] = 0;
     }
@@ -918,47 +938,47 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
 #pragma unroll
                 for (int t = 0; t < V; ++t) val[q][t] *= acc[q][t / 4][t % 4];
         }
-        double vr[G][V];
+        // Lazy rescale: only when a site's largest state falls below 2^-256
+        // (a site's states are spread over the lane groups, so its maximum
+        // takes two cross-lane steps; a lane below the threshold is necessary
+        // for it, so those run only when some lane of the wave is).
+        bool low = false;
 #pragma unroll
-        for (int q = 0; q < G; ++q) {
-            const uint32_t key = round_key(s[q], op.l_node, op.r_node, cat) + 0x9e3779b9u * (8u * (uint32_t)g);
+        for (int q = 0; q < G; ++q)
 #pragma unroll
-            for (int t = 0; t < V; ++t) vr[q][t] = round_cut(val[q][t], key + 0x9e3779b9u * t, T());
-        }
-        // this category's scaler: the per-site maximum over the lane groups
-        int e[G];
+            for (int t = 0; t < V; ++t) low |= __double2hiint(val[q][t]) < kLazyRescaleHi;
+        if (__builtin_amdgcn_ballot_w64(low) != 0) {
 #pragma unroll
-        for (int q = 0; q < G; ++q) {
-            double mx = 0.0;
+            for (int q = 0; q < G; ++q) {
+                const int e = site_exponent<V>(val[q]);
+                const double mx = site_max<V>(val[q]);
+                if (mx < 0x1p-256) {
+                    sc[q] += e;
 #pragma unroll
-            for (int t = 0; t < V; ++t) mx = fmax(mx, val[q][t]);
-            mx = fmax(mx, __shfl_xor(mx, 16, 64));
-            mx = fmax(mx, __shfl_xor(mx, 32, 64));
-            e[q] = 0;
-            if (mx > 0.0) (void)frexp(mx, &e[q]);
-        }
-#pragma unroll
-        for (int q = 0; q < G; ++q) {
-            sc[q] += e[q];
-            T r[V];                                  // stored and carried in the storage precision
-#pragma unroll
-            for (int t = 0; t < V; ++t) {
-                val[q][t] = ldexp(val[q][t], -e[q]);
-                r[t] = (T)ldexp(vr[q][t], -e[q]);
+                    for (int t = 0; t < V; ++t) val[q][t] = ldexp(val[q][t], -e);
+                }
             }
+        }
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
             if (op.out_slot >= 0) {
+                // stored: brought to [0.5, 1) with its own scaler, then rounded
+                // to T with per-state random bits
+                const int e = site_exponent<V>(val[q]);
+                const uint32_t key = round_key(s[q], op.l_node, op.r_node, cat) + 0x9e3779b9u * (8u * (uint32_t)g);
                 T* dst = clv + (size_t)op.out_slot * slot_stride + ((size_t)cat * n_sg + sg0 + q) * V * 64 + lane;
 #pragma unroll
-                for (int t = 0; t < V; ++t) dst[t * 64] = r[t];
-                if (g == 0) scl[((size_t)op.out_slot * C + cat) * m_pad + s[q]] = sc[q];
+                for (int t = 0; t < V; ++t)
+                    dst[t * 64] = (T)round_cut(ldexp(val[q][t], -e), key + 0x9e3779b9u * t, T());
+                if (g == 0) scl[((size_t)op.out_slot * C + cat) * m_pad + s[q]] = sc[q] + e;
             }
             if (op.flags & OP_TO_B) {
 #pragma unroll
-                for (int t = 0; t < V; ++t) cB[q][t] = r[t];
+                for (int t = 0; t < V; ++t) cB[q][t] = val[q][t];
                 sB[q] = sc[q];
             } else {
 #pragma unroll
-                for (int t = 0; t < V; ++t) cA[q][t] = r[t];
+                for (int t = 0; t < V; ++t) cA[q][t] = val[q][t];
                 sA[q] = sc[q];
             }
         }
@@ -966,6 +986,11 @@ __global__ __launch_bounds__(64 * C) void prune_mfma_kernel(
             double lc[G];
 #pragma unroll
             for (int q = 0; q < G; ++q) {
+                // the root partial brought to [0.5, 1) first (as in walk4)
+                const int e = site_exponent<V>(val[q]);
+                sc[q] += e;
+#pragma unroll
+                for (int t = 0; t < V; ++t) val[q][t] = ldexp(val[q][t], -e);
                 lc[q] = 0.0;
 #pragma unroll
                 for (int t = 0; t < V; ++t) lc[q] = fma(roots[4 * t + g], val[q][t], lc[q]);
