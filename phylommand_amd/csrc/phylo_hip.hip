@@ -353,25 +353,38 @@ __device__ inline void children4(const Op& op, const uint8_t* __restrict__ tips,
 // With TL the tip codes of the workgroup's 64 sites come from LDS (tl: taxon
 // t's sites 2b and 2b+1 in the low and high nibble of tl[32 t + b]) and their
 // 0/1 state values from the LDS table tipv (child4).
+// Both children's P matrices of an op (64 SGPRs; their scalar loads
+// overlap), by 32-bit byte offsets from this category's first matrix (three
+// scalar instructions per address instead of five).
+template <int C>
+__device__ __forceinline__ void load_p2(const Op& op, const char* Pcat, double (&Pl)[16], double (&Pr)[16]) {
+    const double* pl = reinterpret_cast<const double*>(Pcat + (uint32_t)op.l_node * (uint32_t)(C * 128));
+    const double* pr = reinterpret_cast<const double*>(Pcat + (uint32_t)op.r_node * (uint32_t)(C * 128));
+#pragma unroll
+    for (int i = 0; i < 16; ++i) { Pl[i] = pl[i]; Pr[i] = pr[i]; }
+}
+
 // One op of walk4 (ops[k] for site s and category cat).  Carry A comes in
 // as cin / sin and the result leaves as cout / sout: walk4 alternates two
 // register sets, so the result is written where the next op reads it, with
-// no copy.
-template <typename T, int C, bool TL>
+// no copy.  With PIPE the op's P arrive in Pl / Pr, and the next op's (nx)
+// are loaded into nPl / nPr once this op's products are done: the P
+// registers are free by then, and the loads run under the rest of the op
+// instead of at the start of the next.
+template <typename T, int C, bool TL, bool PIPE>
 __device__ __forceinline__ void walk4_op(const Op op, const uint8_t* __restrict__ tips, const uint8_t* tl,
                                          const double2* tipv, const T* plane, size_t slot_stride, int* scl,
                                          const char* Pcat, int m_pad, int s, int lane, int cat,
                                          double2 (&xb)[C][2][64], const double (&cin)[4], int sin, int& sB,
-                                         double (&cout)[4], int& sout) {
-    // both children's P up front (64 SGPRs): their scalar loads overlap
-    // (32-bit byte offsets from this category's first matrix: three
-    // scalar instructions per address instead of five)
+                                         double (&cout)[4], int& sout, const double (&Pin_l)[16],
+                                         const double (&Pin_r)[16], const Op& nx, double (&nPl)[16],
+                                         double (&nPr)[16]) {
     double Pl[16], Pr[16];
-    {
-        const double* pl = reinterpret_cast<const double*>(Pcat + (uint32_t)op.l_node * (uint32_t)(C * 128));
-        const double* pr = reinterpret_cast<const double*>(Pcat + (uint32_t)op.r_node * (uint32_t)(C * 128));
+    if constexpr (PIPE) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) { Pl[i] = pl[i]; Pr[i] = pr[i]; }
+        for (int i = 0; i < 16; ++i) { Pl[i] = Pin_l[i]; Pr[i] = Pin_r[i]; }
+    } else {
+        load_p2<C>(op, Pcat, Pl, Pr);
     }
     // child inputs: one straight-line body per (left kind, right kind)
     double L[4];
@@ -388,6 +401,7 @@ __device__ __forceinline__ void walk4_op(const Op op, const uint8_t* __restrict_
         default: PHYLO_SHAPE(3, 0) PHYLO_SHAPE(3, 1) PHYLO_SHAPE(3, 2) PHYLO_SHAPE(3, 3) PHYLO_SHAPE(3, 4)
 #undef PHYLO_SHAPE
     }
+    if constexpr (PIPE) load_p2<C>(nx, Pcat, nPl, nPr);
     // Carried results are FP64 and rescaled lazily: only when the
     // largest entry falls below 2^-256 (rare; a lane-divergent branch
     // costs the others one compare), so an op's children are each at
@@ -449,24 +463,37 @@ __device__ inline void walk4(const Op* __restrict__ ops, int k0, int k1, const u
     int sA2 = 0;
     int k = k0;
     if constexpr (!PAIRS) {
+        double unused[16];
         for (; k < k1; ++k) {
-            walk4_op<T, C, TL>(ops[k], tips, tl, tipv, plane, slot_stride, scl, Pcat, m_pad, s, lane, cat, xb, cA, sA,
-                               sB, cB2, sA2);
+            walk4_op<T, C, TL, false>(ops[k], tips, tl, tipv, plane, slot_stride, scl, Pcat, m_pad, s, lane, cat, xb,
+                                      cA, sA, sB, cB2, sA2, unused, unused, ops[k], unused, unused);
 #pragma unroll
             for (int i = 0; i < 4; ++i) cA[i] = cB2[i];
             sA = sA2;
         }
     } else {
-        // two ops per turn, carry A alternating between cA and cB2
-        for (; k + 1 < k1; k += 2) {
-            walk4_op<T, C, TL>(ops[k], tips, tl, tipv, plane, slot_stride, scl, Pcat, m_pad, s, lane, cat, xb, cA, sA,
-                               sB, cB2, sA2);
-            walk4_op<T, C, TL>(ops[k + 1], tips, tl, tipv, plane, slot_stride, scl, Pcat, m_pad, s, lane, cat, xb,
-                               cB2, sA2, sB, cA, sA);
+        // two ops per turn, carry A alternating between cA and cB2, and the
+        // P matrices between PA and PB.  Each op brings in the next op
+        // (its fields loaded one op ahead, its P once the products are
+        // done), so the next op starts without a scalar-load wait.  The op
+        // buffer holds a zero op after the last program (node 0's P are
+        // loaded for it, never used), so ops[k1] is always readable.
+        double PA_l[16], PA_r[16], PB_l[16], PB_r[16];
+        const Op* op = ops + k0;
+        const Op* const op_end = ops + k1;
+        Op cur = op[0];
+        load_p2<C>(cur, Pcat, PA_l, PA_r);
+        for (; op + 1 < op_end; op += 2) {
+            const Op nx = op[1];
+            walk4_op<T, C, TL, true>(cur, tips, tl, tipv, plane, slot_stride, scl, Pcat, m_pad, s, lane, cat, xb,
+                                     cA, sA, sB, cB2, sA2, PA_l, PA_r, nx, PB_l, PB_r);
+            cur = op[2];
+            walk4_op<T, C, TL, true>(nx, tips, tl, tipv, plane, slot_stride, scl, Pcat, m_pad, s, lane, cat,
+                                     xb, cB2, sA2, sB, cA, sA, PB_l, PB_r, cur, PA_l, PA_r);
         }
-        if (k < k1) {       // an odd op count: the last op, its result back in cA
-            walk4_op<T, C, TL>(ops[k], tips, tl, tipv, plane, slot_stride, scl, Pcat, m_pad, s, lane, cat, xb, cA, sA,
-                               sB, cB2, sA2);
+        if (op < op_end) {  // an odd op count: the last op, its result back in cA
+            walk4_op<T, C, TL, true>(cur, tips, tl, tipv, plane, slot_stride, scl, Pcat, m_pad, s, lane, cat, xb,
+                                     cA, sA, sB, cB2, sA2, PA_l, PA_r, cur, PB_l, PB_r);
 #pragma unroll
             for (int i = 0; i < 4; ++i) cA[i] = cB2[i];
             sA = sA2;
@@ -1225,6 +1252,17 @@ hipError_t ensure(T*& ptr, size_t& have, size_t need) {
     return e;
 }
 
+// Copies a program's ops to the device followed by one zero op: the DNA
+// walker reads the op after the one it runs (up to ops[end] of the last
+// program) to load its P matrices early.
+hipError_t upload_ops(Op*& d, size_t& have, const std::vector<Op>& ops, hipStream_t st) {
+    static const Op pad{};
+    hipError_t e = ensure(d, have, (ops.size() + 1) * sizeof(Op));
+    if (e == hipSuccess) e = hipMemcpyAsync(d, ops.data(), ops.size() * sizeof(Op), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d + ops.size(), &pad, sizeof(Op), hipMemcpyHostToDevice, st);
+    return e;
+}
+
 // Host staging area of n doubles, free to overwrite once the copy that last
 // read it has completed.
 hipError_t stage_acquire(phylo_ctx::Stage& st, size_t n, double** out) {
@@ -1678,9 +1716,8 @@ int prepare(phylo_ctx* c) {
         int rc = compile_full_program(c);
         if (rc) return rc;
         if (!root_ops_last(c->ops, c->prog_begin)) return fail(c, PHYLO_ERR_STATE, "internal: root op not last");
-        HIPCHK(c, ensure(c->d_ops, c->ops_bytes, c->ops.size() * sizeof(Op)));
+        HIPCHK(c, upload_ops(c->d_ops, c->ops_bytes, c->ops, c->stream));
         HIPCHK(c, ensure(c->d_prog, c->prog_bytes, c->prog_begin.size() * sizeof(int)));
-        HIPCHK(c, hipMemcpyAsync(c->d_ops, c->ops.data(), c->ops.size() * sizeof(Op), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->d_prog, c->prog_begin.data(), c->prog_begin.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         c->dirty_topo = false;
@@ -2364,9 +2401,8 @@ int phylo_nni_score(phylo_ctx* c, int mode, int max_variants, double* lnl, int32
     const int n_prog = (int)which.size();
     if (n_prog == 0) return PHYLO_OK;
     if (!root_ops_last(ops, begin)) return fail(c, PHYLO_ERR_STATE, "internal: root op not last");
-    HIPCHK(c, ensure(c->d_nni_ops, c->nni_ops_bytes, ops.size() * sizeof(Op)));
+    HIPCHK(c, upload_ops(c->d_nni_ops, c->nni_ops_bytes, ops, c->stream));
     HIPCHK(c, ensure(c->d_nni_prog, c->nni_prog_bytes, begin.size() * sizeof(int)));
-    HIPCHK(c, hipMemcpyAsync(c->d_nni_ops, ops.data(), ops.size() * sizeof(Op), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_nni_prog, begin.data(), begin.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
     std::vector<double> tot(n_prog);
     rc = run_programs(c, c->d_nni_ops, c->d_nni_prog, n_prog, tot.data(), nullptr);
@@ -2585,7 +2621,7 @@ extern "C" int phylo_loglik_noalloc(phylo_ctx* c, int n_taxa, int n_sites, int n
         return at;
     };
     const size_t o_tips = take((size_t)n_taxa * m_pad);
-    const size_t o_ops = take(ops.size() * sizeof(Op));
+    const size_t o_ops = take((ops.size() + 1) * sizeof(Op));       // and a zero op after the program
     const size_t o_prog = take(2 * sizeof(int));
     const size_t o_eig = take(n_eig * sizeof(double));
     const size_t o_roots = take(n_roots * sizeof(double));
