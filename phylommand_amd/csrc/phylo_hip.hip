@@ -58,7 +58,10 @@ constexpr int kSplitAllBelowPerCu = 4;
 constexpr int kMfmaGroups = 2;
 
 enum SrcKind : int { SRC_TIP = 0, SRC_CLV = 1, SRC_CARRY_A = 2, SRC_CARRY_B = 3, SRC_NONE = 4 };
-enum OpFlags : int { OP_ROOT = 1, OP_TO_B = 2 };
+// OP_SWAP: the children were swapped into kind order (order_children); the
+// stochastic rounding of a stored result is keyed by them in their original
+// order.
+enum OpFlags : int { OP_ROOT = 1, OP_TO_B = 2, OP_SWAP = 4 };
 
 struct alignas(16) Op {
     int l_node, r_node;   // child node ids: P matrices are per (child node, category)
@@ -128,10 +131,14 @@ __device__ inline double round_cut(double x, uint32_t bits, float) {
 __device__ inline double round_cut(double x, uint32_t, double) { return x; }
 
 // Random bits for the partials of one (site, node, category); state i uses
-// key + i * 0x9e3779b9.  The node is identified by its two children.
+// key + i * 0x9e3779b9.  The node is identified by its two children, in the
+// order the program builder gave them.
 __device__ inline uint32_t round_key(int s, int l_node, int r_node, int cat) {
     return mix32((uint32_t)s * 0x9e3779b1u ^ ((uint32_t)l_node * 0x85ebca77u + (uint32_t)r_node * 0x27d4eb2fu +
                                               (uint32_t)cat * 0xc2b2ae3du));
+}
+__device__ inline uint32_t round_key(int s, const Op& op, int cat) {
+    return (op.flags & OP_SWAP) ? round_key(s, op.r_node, op.l_node, cat) : round_key(s, op.l_node, op.r_node, cat);
 }
 
 __device__ inline double wave_sum(double v) {
@@ -395,10 +402,11 @@ __device__ __forceinline__ void walk4_op(const Op op, const uint8_t* __restrict_
         children4<A, B, T, C, TL>(op, tips, tl, tipv, plane, slot_stride, scl, m_pad, s, lane, cat, cin, sin, xb, \
                                   sB, Pl, Pr, L, sc);                                                     \
         break;
-        PHYLO_SHAPE(0, 0) PHYLO_SHAPE(0, 1) PHYLO_SHAPE(0, 2) PHYLO_SHAPE(0, 3) PHYLO_SHAPE(0, 4)
-        PHYLO_SHAPE(1, 0) PHYLO_SHAPE(1, 1) PHYLO_SHAPE(1, 2) PHYLO_SHAPE(1, 3) PHYLO_SHAPE(1, 4)
-        PHYLO_SHAPE(2, 0) PHYLO_SHAPE(2, 1) PHYLO_SHAPE(2, 2) PHYLO_SHAPE(2, 3) PHYLO_SHAPE(2, 4)
-        default: PHYLO_SHAPE(3, 0) PHYLO_SHAPE(3, 1) PHYLO_SHAPE(3, 2) PHYLO_SHAPE(3, 3) PHYLO_SHAPE(3, 4)
+        // children in kind order: carry A, tip, carry B, stored, none
+        PHYLO_SHAPE(2, 0) PHYLO_SHAPE(2, 3) PHYLO_SHAPE(2, 1) PHYLO_SHAPE(2, 4)
+        PHYLO_SHAPE(0, 0) PHYLO_SHAPE(0, 3) PHYLO_SHAPE(0, 1) PHYLO_SHAPE(0, 4)
+        PHYLO_SHAPE(3, 1) PHYLO_SHAPE(3, 4) PHYLO_SHAPE(1, 1)
+        default: PHYLO_SHAPE(1, 4)
 #undef PHYLO_SHAPE
     }
     if constexpr (PIPE) load_p2<C>(nx, Pcat, nPl, nPr);
@@ -422,7 +430,7 @@ __device__ __forceinline__ void walk4_op(const Op op, const uint8_t* __restrict_
         // stores what it reads back), so the rounding sits here, on the
         // few stored ones
         const int e = max_exponent(L);
-        const uint32_t key = round_key(s, op.l_node, op.r_node, cat);
+        const uint32_t key = round_key(s, op, cat);
         T r[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) r[i] = (T)round_cut(ldexp(L[i], -e), key + 0x9e3779b9u * i, T());
@@ -992,7 +1000,7 @@ __global__ __launch_bounds__(64 * C, (V <= 5 ? 3 : 1)) void prune_mfma_kernel(
                 // stored: brought to [0.5, 1) with its own scaler, then rounded
                 // to T with per-state random bits
                 const int e = site_exponent<V>(val[q]);
-                const uint32_t key = round_key(s[q], op.l_node, op.r_node, cat) + 0x9e3779b9u * (8u * (uint32_t)g);
+                const uint32_t key = round_key(s[q], op, cat) + 0x9e3779b9u * (8u * (uint32_t)g);
                 T* dst = clv + (size_t)op.out_slot * slot_stride + ((size_t)cat * n_sg + sg0 + q) * V * 64 + lane;
 #pragma unroll
                 for (int t = 0; t < V; ++t)
@@ -1252,11 +1260,30 @@ hipError_t ensure(T*& ptr, size_t& have, size_t need) {
     return e;
 }
 
-// Copies a program's ops to the device followed by one zero op: the DNA
-// walker reads the op after the one it runs (up to ops[end] of the last
-// program) to load its P matrices early.
-hipError_t upload_ops(Op*& d, size_t& have, const std::vector<Op>& ops, hipStream_t st) {
+// Puts each op's children in kind order (carry A, tip, carry B, stored
+// partial, none), swapping them where needed and marking the op OP_SWAP, so
+// the walkers meet few child shapes (a Yule tree's ops take four) and the
+// DNA walker's products start with the child already in registers while a
+// tip's LDS reads are in flight.  No value changes: the children's products
+// are multiplied elementwise, which commutes, and round_key undoes the swap.
+// Idempotent.
+void order_children(std::vector<Op>& ops) {
+    static const int rank[5] = {1, 3, 0, 2, 4};    // by SrcKind: TIP, CLV, CARRY_A, CARRY_B, NONE
+    for (Op& o : ops)
+        if (rank[o.r_kind] < rank[o.l_kind]) {
+            std::swap(o.l_node, o.r_node);
+            std::swap(o.l_kind, o.r_kind);
+            std::swap(o.l_idx, o.r_idx);
+            o.flags ^= OP_SWAP;
+        }
+}
+
+// Copies a program's ops to the device, children in kind order, followed by
+// one zero op: the DNA walker reads the op after the one it runs (up to
+// ops[end] of the last program) to load its P matrices early.
+hipError_t upload_ops(Op*& d, size_t& have, std::vector<Op>& ops, hipStream_t st) {
     static const Op pad{};
+    order_children(ops);
     hipError_t e = ensure(d, have, (ops.size() + 1) * sizeof(Op));
     if (e == hipSuccess) e = hipMemcpyAsync(d, ops.data(), ops.size() * sizeof(Op), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(d + ops.size(), &pad, sizeof(Op), hipMemcpyHostToDevice, st);
@@ -2645,6 +2672,7 @@ extern "C" int phylo_loglik_noalloc(phylo_ctx* c, int n_taxa, int n_sites, int n
         for (int s = 0; s < n_sites; ++s) row[s] = (uint8_t)(masks[(size_t)t * n_sites + s] & valid);
         std::fill(row + n_sites, row + m_pad, (uint8_t)0x0F);     // padding sites: all states
     }
+    order_children(ops);
     std::memcpy(in.data() + o_ops, ops.data(), ops.size() * sizeof(Op));
     const int prog[2] = {0, (int)ops.size()};
     std::memcpy(in.data() + o_prog, prog, sizeof prog);
@@ -3092,6 +3120,7 @@ int phylo_batch_finalize(phylo_batch* b, int autotune) {
         std::vector<double>().swap(I.P);
     }
     HIPCHK(b, hipMemcpy(b->d_inst, hinst.data(), hinst.size() * sizeof(BatchInst), hipMemcpyHostToDevice));
+    order_children(ops);
     HIPCHK(b, hipMemcpy(b->d_ops, ops.data(), ops.size() * sizeof(Op), hipMemcpyHostToDevice));
     HIPCHK(b, hipMemcpy(b->d_prog, prog_begin.data(), prog_begin.size() * sizeof(int), hipMemcpyHostToDevice));
     const int n_out = (int)b->inst.size() + b->n_var;
