@@ -975,13 +975,17 @@ __global__ __launch_bounds__(64 * C, (V <= 5 ? 3 : 1)) void prune_mfma_kernel(
         }
         // Lazy rescale: only when a site's largest state falls below 2^-256
         // (a site's states are spread over the lane groups, so its maximum
-        // takes two cross-lane steps; a lane below the threshold is necessary
-        // for it, so those run only when some lane of the wave is).
+        // takes two cross-lane steps; every lane holding part of such a site
+        // has its own largest state below the threshold, so those steps run
+        // only when some lane of the wave has).
         bool low = false;
 #pragma unroll
-        for (int q = 0; q < G; ++q)
+        for (int q = 0; q < G; ++q) {
+            int h = __double2hiint(val[q][0]);     // the lane's largest state of the site, as a high word
 #pragma unroll
-            for (int t = 0; t < V; ++t) low |= __double2hiint(val[q][t]) < kLazyRescaleHi;
+            for (int t = 1; t < V; ++t) h = max(h, __double2hiint(val[q][t]));
+            low |= h < kLazyRescaleHi;
+        }
         if (__builtin_amdgcn_ballot_w64(low) != 0) {
 #pragma unroll
             for (int q = 0; q < G; ++q) {
