@@ -274,18 +274,20 @@ __device__ inline int max_hi_word(const double (&L)[4]) {
 constexpr int kLazyRescaleHi = (1023 - 256) << 20;
 
 // One child's operand for a walker op, by source kind (compile-time).
-// With TL the tip's state set is read from the workgroup's LDS copy and its
-// four 0/1 values from tipv, a 16-entry LDS table (entry 2 code + h: states
-// 2h and 2h + 1 of code): two LDS reads where the bit-by-bit decode takes
-// four bit-field extracts and four conversions.
+// With TL the tip's state set comes from the workgroup's LDS copy, one byte
+// per site holding 16 x code, which is the byte offset of the code's entry
+// in tipv: states 0, 1 at tipv[code], states 2, 3 at tipv[16 + code] (two
+// LDS reads addressed by the byte itself, where the bit-by-bit decode takes
+// four bit-field extracts and four conversions).
 template <int K, typename T, int C, bool TL>
 __device__ inline void child4(int idx, const uint8_t* __restrict__ tips, const uint8_t* tl, const double2* tipv,
                               const T* plane, size_t slot_stride, const int* scl, int m_pad, int s, int lane, int cat,
                               const double (&cA)[4], int sA, double2 (&xb)[C][2][64], int sB, double (&x)[4], int& sc) {
     if (K == SRC_TIP) {
         if (TL) {
-            const unsigned code = (tl[idx * 32 + (lane >> 1)] >> ((lane & 1) * 4)) & 15u;
-            const double2 a = tipv[2 * code], b = tipv[2 * code + 1];
+            const unsigned off = tl[idx * 64 + lane];
+            const double2 a = *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(tipv) + off);
+            const double2 b = *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(tipv + 16) + off);
             x[0] = a.x; x[1] = a.y; x[2] = b.x; x[3] = b.y;
         } else {
             const unsigned code = tips[(size_t)idx * m_pad + s];
@@ -357,9 +359,9 @@ __device__ inline void children4(const Op& op, const uint8_t* __restrict__ tips,
 // root_lc = w_cat sum_i rw_i L_i and root_sc, the category's scaler, so that
 // its share of the site likelihood is root_lc 2^root_sc (the root is read
 // from carry A after the loop, so the loop carries no root values).
-// With TL the tip codes of the workgroup's 64 sites come from LDS (tl: taxon
-// t's sites 2b and 2b+1 in the low and high nibble of tl[32 t + b]) and their
-// 0/1 state values from the LDS table tipv (child4).
+// With TL the tip codes of the workgroup's 64 sites come from LDS (tl: site
+// j of taxon t at tl[64 t + j], as 16 x code) and their 0/1 state values from
+// the LDS table tipv (child4).
 // Both children's P matrices of an op (64 SGPRs; their scalar loads
 // overlap), by 32-bit byte offsets from this category's first matrix (three
 // scalar instructions per address instead of five).
@@ -545,8 +547,10 @@ __device__ inline double combine_categories(double lc, int sc, int cat, int lane
     return log(lc) + sc * kLn2;
 }
 
-// Taxa whose tip codes a DNA workgroup stages in LDS (4 bits per site).
+// Taxa whose tip codes a DNA workgroup stages in LDS (one byte per site,
+// sized at launch).
 constexpr int kTipLdsTaxa = 256;
+inline size_t tip_lds_bytes(int n_taxa) { return (size_t)n_taxa * 64; }
 
 // Plain launch (prog_main < 0): grid = (m_pad / 64, n_programs); workgroup
 // (x, y) runs ops[prog_begin[y] .. prog_begin[y+1]) over site block x.
@@ -569,7 +573,7 @@ __global__ __launch_bounds__(64 * C) void prune4_kernel(
     __shared__ int xsc[C][64];
     __shared__ double xroot[C][64];
     __shared__ double2 xb[C][2][64];
-    __shared__ uint8_t tl[TL ? kTipLdsTaxa * 32 : 1];
+    extern __shared__ __align__(16) uint8_t tl[];   // TL: n_taxa x 64 bytes (launch size)
     __shared__ double2 tipv[TL ? 32 : 1];
     const int lane = threadIdx.x & 63;
     const int cat = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -590,13 +594,13 @@ __global__ __launch_bounds__(64 * C) void prune4_kernel(
     const int s = bx * 64 + lane;
     if (TL) {
         if (threadIdx.x < 32) {          // 0/1 values of states 2h, 2h + 1 of each code
-            const unsigned code = threadIdx.x >> 1, h = (threadIdx.x & 1) * 2;
+            const unsigned code = threadIdx.x & 15, h = (threadIdx.x >> 4) * 2;
             tipv[threadIdx.x] = make_double2((double)((code >> h) & 1u), (double)((code >> (h + 1)) & 1u));
         }
         const uint8_t* blk = tips + (size_t)bx * 64;
-        for (int i = threadIdx.x; i < n_taxa * 32; i += 64 * C) {
-            const uint32_t pr = *reinterpret_cast<const uint16_t*>(blk + (size_t)(i >> 5) * m_pad + 2 * (i & 31));
-            tl[i] = (uint8_t)((pr & 15u) | ((pr >> 4) & 0xF0u));
+        for (int i = threadIdx.x; i < n_taxa * 16; i += 64 * C) {      // four sites a thread
+            const uint32_t w = *reinterpret_cast<const uint32_t*>(blk + (size_t)(i >> 4) * m_pad + 4 * (i & 15));
+            reinterpret_cast<uint32_t*>(tl)[i] = (w << 4) & 0xF0F0F0F0u;
         }
         __syncthreads();
     }
@@ -1569,9 +1573,10 @@ bool build_split_programs(phylo_ctx* c, int max_chunks, bool split_all) {
 // context's alphabet, precision and rate categories (the occupancy API on the
 // kernel instance the launch uses); 0 if unknown.
 template <typename T, int C>
-int resident_dna(const phylo_ctx*) {
+int resident_dna(const phylo_ctx* c) {
     int n = 0;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, prune4_kernel<T, C, true>, 64 * C, 0) == hipSuccess ? n : 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, prune4_kernel<T, C, true>, 64 * C,
+                                                        tip_lds_bytes(c->n_taxa)) == hipSuccess ? n : 0;
 }
 template <typename T, int C, int V>
 int resident_wide(const phylo_ctx*) {
@@ -1800,7 +1805,8 @@ void launch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
         if (c->tail_split && c->split_K > 0 && grid.y == 1 && tail > 0) {
             // the main blocks and the tail's pieces, then the tail's top program
             const int base = c->split_base[c->run_full ? 0 : 1], K = c->split_K;
-            hipLaunchKernelGGL((prune4_kernel<T, C, true>), dim3(main_b + K * tail), dim3(64 * C), 0, c->stream,
+            hipLaunchKernelGGL((prune4_kernel<T, C, true>), dim3(main_b + K * tail), dim3(64 * C),
+                               tip_lds_bytes(c->n_taxa), c->stream,
                                c->run_ops, c->d_prog, tips, clv, c->d_scl, c->d_P, c->d_roots, sw, site_lnl,
                                c->d_partial, c->m, c->m_pad, c->n_taxa, main_b, c->run_full ? 0 : 1, 0, base,
                                main_b, tail, nullptr, nullptr);
@@ -1811,7 +1817,8 @@ void launch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
                                c->m_pad, c->n_taxa, tail, base + K, main_b, 0, 0, 1, fold, c->h_total_dev);
             c->folded = fold != nullptr;
         } else {
-            hipLaunchKernelGGL((prune4_kernel<T, C, true>), grid, dim3(64 * C), 0, c->stream, c->run_ops, c->run_prog,
+            hipLaunchKernelGGL((prune4_kernel<T, C, true>), grid, dim3(64 * C), tip_lds_bytes(c->n_taxa), c->stream,
+                               c->run_ops, c->run_prog,
                                tips, clv, c->d_scl, c->d_P, c->d_roots, sw, site_lnl, c->d_partial, c->m, c->m_pad,
                                c->n_taxa, (int)grid.x, -1, 0, 0, 0, 1, nullptr, nullptr);
         }
@@ -2571,7 +2578,8 @@ void launch_noalloc(hipStream_t st, int nblk, bool tip_lds, const Op* ops, const
                     float* clv, int* scl, const double* P, const double* roots, double* site, double* partial, int m,
                     int m_pad, int n_taxa) {
     if (tip_lds)
-        hipLaunchKernelGGL((prune4_kernel<float, C, true>), dim3(nblk), dim3(64 * C), 0, st, ops, prog, tips, clv, scl,
+        hipLaunchKernelGGL((prune4_kernel<float, C, true>), dim3(nblk), dim3(64 * C), tip_lds_bytes(n_taxa), st, ops,
+                           prog, tips, clv, scl,
                            P, roots, nullptr, site, partial, m, m_pad, n_taxa, nblk, -1, 0, 0, 0, 1, nullptr, nullptr);
     else
         hipLaunchKernelGGL((prune4_kernel<float, C, false>), dim3(nblk), dim3(64 * C), 0, st, ops, prog, tips, clv,
