@@ -61,7 +61,14 @@ enum SrcKind : int { SRC_TIP = 0, SRC_CLV = 1, SRC_CARRY_A = 2, SRC_CARRY_B = 3,
 // OP_SWAP: the children were swapped into kind order (order_children); the
 // stochastic rounding of a stored result is keyed by them in their original
 // order.
-enum OpFlags : int { OP_ROOT = 1, OP_TO_B = 2, OP_SWAP = 4 };
+// OP_SH_*: one-hot hints for the four commonest child shapes (children in
+// kind order), set by order_children; the DNA walker tests them before its
+// switch.
+enum OpFlags : int {
+    OP_ROOT = 1, OP_TO_B = 2, OP_SWAP = 4,
+    OP_SH_AT = 8, OP_SH_TT = 16, OP_SH_AB = 32, OP_SH_AC = 64,
+    OP_SH_ALL = OP_SH_AT | OP_SH_TT | OP_SH_AB | OP_SH_AC
+};
 
 struct alignas(16) Op {
     int l_node, r_node;   // child node ids: P matrices are per (child node, category)
@@ -395,18 +402,32 @@ __device__ __forceinline__ void walk4_op(const Op op, const uint8_t* __restrict_
     } else {
         load_p2<C>(op, Pcat, Pl, Pr);
     }
-    // child inputs: one straight-line body per (left kind, right kind)
+    // child inputs: one straight-line body per (left kind, right kind); the
+    // commonest shapes by their one-hot hint bits first (single-bit tests,
+    // which the compiler keeps apart from the switch's compare tree)
     double L[4];
     int sc = 0;
-    switch (op.l_kind * 5 + op.r_kind) {
+    if (op.flags & OP_SH_AT)
+        children4<SRC_CARRY_A, SRC_TIP, T, C, TL>(op, tips, tl, tipv, plane, slot_stride, scl, m_pad, s, lane, cat,
+                                                  cin, sin, xb, sB, Pl, Pr, L, sc);
+    else if (op.flags & OP_SH_TT)
+        children4<SRC_TIP, SRC_TIP, T, C, TL>(op, tips, tl, tipv, plane, slot_stride, scl, m_pad, s, lane, cat, cin,
+                                              sin, xb, sB, Pl, Pr, L, sc);
+    else if (op.flags & OP_SH_AB)
+        children4<SRC_CARRY_A, SRC_CARRY_B, T, C, TL>(op, tips, tl, tipv, plane, slot_stride, scl, m_pad, s, lane, cat,
+                                                      cin, sin, xb, sB, Pl, Pr, L, sc);
+    else if (op.flags & OP_SH_AC)
+        children4<SRC_CARRY_A, SRC_CLV, T, C, TL>(op, tips, tl, tipv, plane, slot_stride, scl, m_pad, s, lane, cat,
+                                                  cin, sin, xb, sB, Pl, Pr, L, sc);
+    else switch (op.l_kind * 5 + op.r_kind) {
 #define PHYLO_SHAPE(A, B)                                                                                   \
     case A * 5 + B:                                                                                       \
         children4<A, B, T, C, TL>(op, tips, tl, tipv, plane, slot_stride, scl, m_pad, s, lane, cat, cin, sin, xb, \
                                   sB, Pl, Pr, L, sc);                                                     \
         break;
-        // children in kind order: carry A, tip, carry B, stored, none
-        PHYLO_SHAPE(2, 0) PHYLO_SHAPE(2, 3) PHYLO_SHAPE(2, 1) PHYLO_SHAPE(2, 4)
-        PHYLO_SHAPE(0, 0) PHYLO_SHAPE(0, 3) PHYLO_SHAPE(0, 1) PHYLO_SHAPE(0, 4)
+        // the other shapes of children in kind order (carry A, tip, carry B,
+        // stored, none)
+        PHYLO_SHAPE(2, 4) PHYLO_SHAPE(0, 3) PHYLO_SHAPE(0, 1) PHYLO_SHAPE(0, 4)
         PHYLO_SHAPE(3, 1) PHYLO_SHAPE(3, 4) PHYLO_SHAPE(1, 1)
         default: PHYLO_SHAPE(1, 4)
 #undef PHYLO_SHAPE
@@ -1277,13 +1298,19 @@ hipError_t ensure(T*& ptr, size_t& have, size_t need) {
 // Idempotent.
 void order_children(std::vector<Op>& ops) {
     static const int rank[5] = {1, 3, 0, 2, 4};    // by SrcKind: TIP, CLV, CARRY_A, CARRY_B, NONE
-    for (Op& o : ops)
+    for (Op& o : ops) {
         if (rank[o.r_kind] < rank[o.l_kind]) {
             std::swap(o.l_node, o.r_node);
             std::swap(o.l_kind, o.r_kind);
             std::swap(o.l_idx, o.r_idx);
             o.flags ^= OP_SWAP;
         }
+        o.flags &= ~OP_SH_ALL;
+        if (o.l_kind == SRC_CARRY_A && o.r_kind == SRC_TIP) o.flags |= OP_SH_AT;
+        if (o.l_kind == SRC_TIP && o.r_kind == SRC_TIP) o.flags |= OP_SH_TT;
+        if (o.l_kind == SRC_CARRY_A && o.r_kind == SRC_CARRY_B) o.flags |= OP_SH_AB;
+        if (o.l_kind == SRC_CARRY_A && o.r_kind == SRC_CLV) o.flags |= OP_SH_AC;
+    }
 }
 
 // Copies a program's ops to the device, children in kind order, followed by
