@@ -1557,11 +1557,14 @@ bool build_split_programs(phylo_ctx* c, int max_chunks, bool split_all) {
         for (const auto& ch : chunks) big = std::max(big, ch.size());
         if (chunks.size() < 2 || top.size() > 16 || (int)chunks.size() > max_chunks) continue;
         // Tail split: the top program runs alone, about 0.7 us per op; a
-        // chunk shares a CU with main blocks and costs little more as it
-        // grows (C2 tree: top 2 / chunks <= 56 ops 0.137 ms, top 6 / <= 42
-        // 0.139, top 9 / <= 20 0.141, top 15 / <= 15 0.145).  Every block
-        // split: the longest chunk and then the top program form the chain.
-        const double cost = split_all ? (double)big + (double)top.size() : (double)top.size() + (double)big / 40.0;
+        // chunk shares a CU with main blocks and costs nothing until it is
+        // longer than about 0.4 of the whole walk, then about 0.5 us per op
+        // (C2 tree, 127 ops, kernel span: top 1 / longest chunk 70 ops
+        // 0.1134 ms, 2 / 56 0.1066, 3 / 49 0.1062, 4 / 44 0.1063, 7 / 34
+        // 0.1074, 8 / 28 0.1083, 9 / 20 0.1089).  Every block split: the
+        // longest chunk and then the top program form the chain.
+        const double over = std::max(0.0, (double)big - 0.4 * ni);
+        const double cost = split_all ? (double)big + (double)top.size() : (double)top.size() + 0.7 * over;
         if (cost < best_cost) { best_cost = cost; best_L = L; }
     }
     if (best_L < 0) return false;
