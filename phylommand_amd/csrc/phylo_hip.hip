@@ -429,7 +429,7 @@ __device__ __forceinline__ void walk4_op(const Op op, const uint8_t* __restrict_
         // stored, none)
         PHYLO_SHAPE(2, 4) PHYLO_SHAPE(0, 3) PHYLO_SHAPE(0, 1) PHYLO_SHAPE(0, 4)
         PHYLO_SHAPE(3, 1) PHYLO_SHAPE(3, 4) PHYLO_SHAPE(1, 1)
-        default: PHYLO_SHAPE(1, 4)
+        default: PHYLO_SHAPE(1, 4)      // (no other shape occurs: one carry A and one carry B at a time)
 #undef PHYLO_SHAPE
     }
     if constexpr (PIPE) load_p2<C>(nx, Pcat, nPl, nPr);
