@@ -548,8 +548,9 @@ __device__ inline void walk4(const Op* __restrict__ ops, int k0, int k1, const u
 
 // Site lnL from the categories' root sums (lc, sc) held by the C waves of a
 // workgroup (wave = category): the sums meet in LDS at their largest
-// exponent.  Every wave returns the same value.
-template <int C>
+// exponent.  Every wave returns the same value.  REUSE: the caller writes
+// xroot / xsc again later, so the waves meet once more after reading.
+template <int C, bool REUSE = true>
 __device__ inline double combine_categories(double lc, int sc, int cat, int lane, double (&xroot)[C][64],
                                             int (&xsc)[C][64]) {
     if (C > 1) {
@@ -562,7 +563,7 @@ __device__ inline double combine_categories(double lc, int sc, int cat, int lane
         lc = 0.0;
 #pragma unroll
         for (int c = 0; c < C; ++c) lc += ldexp(xroot[c][lane], xsc[c][lane] - M);
-        __syncthreads();
+        if (REUSE) __syncthreads();
         sc = M;
     }
     return log(lc) + sc * kLn2;
@@ -630,13 +631,20 @@ __global__ __launch_bounds__(64 * C) void prune4_kernel(
     walk4<T, C, TL>(ops, prog_begin[prog], prog_begin[prog + 1], tips, tl, tipv, clv, scl, P, roots, m_pad, s, lane,
                     cat, xb, lc, sc);
     if (!has_root) return;    // a piece: its stores are its result (uniform per workgroup)
-    const double lnl = combine_categories<C>(lc, sc, cat, lane, xroot, xsc);
+    const double lnl = combine_categories<C, false>(lc, sc, cat, lane, xroot, xsc);
     const bool live = s < m && cat == 0;
     if (live && site_lnl) site_lnl[(size_t)y * m + s] = lnl;
     const double w = live ? (site_w ? site_w[s] : 1.0) : 0.0;
-    block_sum_store(live ? w * lnl : 0.0,
-                    partial + (prog_main >= 0 ? (size_t)bx : (size_t)y * gridDim.x + blockIdx.x));
-    if (64 * C == kSumThreads && fold_counter) fold_total(partial, blk0 + n_main, fold_counter, fold_out);
+    double* dst = partial + (prog_main >= 0 ? (size_t)bx : (size_t)y * gridDim.x + blockIdx.x);
+    if (64 * C == kSumThreads && fold_counter) {
+        block_sum_store(live ? w * lnl : 0.0, dst);
+        fold_total(partial, blk0 + n_main, fold_counter, fold_out);
+    } else if (cat == 0) {
+        // only the category-0 wave holds site values: its sum is the
+        // block's (the other waves' zeros would add nothing)
+        const double v = wave_sum(live ? w * lnl : 0.0);
+        if (lane == 0) *dst = v;
+    }
 }
 
 // Batched scoring: many independent problems (and their NNI programs) in one
