@@ -63,7 +63,7 @@ enum SrcKind : int { SRC_TIP = 0, SRC_CLV = 1, SRC_CARRY_A = 2, SRC_CARRY_B = 3,
 // order.
 // OP_SH_*: one-hot hints for the four commonest child shapes (children in
 // kind order), set by order_children; the DNA walker tests them before its
-// switch.
+// switch, the matrix walker runs a specialised op for each.
 enum OpFlags : int {
     OP_ROOT = 1, OP_TO_B = 2, OP_SWAP = 4,
     OP_SH_AT = 8, OP_SH_TT = 16, OP_SH_AB = 32, OP_SH_AC = 64,
@@ -839,7 +839,7 @@ __device__ inline void mfma_child(int kind, int idx, int node, int cat, const in
                                   const uint32_t* __restrict__ tips, const uint32_t* tl, const T* clv,
                                   size_t slot_stride,
                                   const int* scl, const double* __restrict__ Pm, const double* __restrict__ PTm,
-                                  const double (&cA)[G][V], const int (&sA)[G], const double (&cB)[G][V],
+                                  const double (&cA)[G][V], const int (&sA)[G], const double* cB,
                                   const int (&sB)[G], mfma_d4 (&acc)[G][2], int (&sc)[G]) {
     constexpr int TN = V > 4 ? 2 : 1;
     double x[G][V];
@@ -886,7 +886,7 @@ __device__ inline void mfma_child(int kind, int idx, int node, int cat, const in
 #pragma unroll
         for (int q = 0; q < G; ++q) {
 #pragma unroll
-            for (int t = 0; t < V; ++t) x[q][t] = to_d(cB[q][t]);
+            for (int t = 0; t < V; ++t) x[q][t] = cB[(q * V + t) * 64];
             sc[q] += sB[q];
         }
     }
@@ -920,6 +920,170 @@ __device__ inline void mfma_child(int kind, int idx, int node, int cat, const in
     }
 }
 
+// The P operand of one child for every row tile and K-step (Pm order: tile
+// T, K-step t at register T V + t).
+template <int C, int V>
+__device__ __forceinline__ void load_pm(int node, int cat, int lane, const double* __restrict__ Pm,
+                                        double (&pf)[(V > 4 ? 2 : 1) * V]) {
+    constexpr int TN = V > 4 ? 2 : 1;
+    const double* pm = Pm + ((size_t)node * C + cat) * TN * V * 64 + lane;
+#pragma unroll
+    for (int i = 0; i < TN * V; ++i) pf[i] = pm[i * 64];
+}
+
+// r = P . x for the G site groups with the P operand in registers: the same
+// matrix instructions in the same order as mfma_child; r[q][t] is state
+// 4t + g of group q's site.
+template <int V, int G>
+__device__ __forceinline__ void mfma_products(const double (&pf)[(V > 4 ? 2 : 1) * V], const double (&x)[G][V],
+                                              double (&r)[G][V]) {
+    constexpr int TN = V > 4 ? 2 : 1;
+    mfma_d4 acc[G][2];
+#pragma unroll
+    for (int T2 = 0; T2 < TN; ++T2) {
+        if (V == 5 && T2 == 1) {
+            double d[G];
+#pragma unroll
+            for (int q = 0; q < G; ++q) d[q] = 0.0;
+#pragma unroll
+            for (int t = 0; t < V; ++t)
+#pragma unroll
+                for (int q = 0; q < G; ++q) d[q] = __builtin_amdgcn_mfma_f64_4x4x4f64(pf[V + t], x[q][t], d[q], 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < G; ++q) acc[q][1][0] = d[q];
+            continue;
+        }
+#pragma unroll
+        for (int q = 0; q < G; ++q) acc[q][T2] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < V; ++t)
+#pragma unroll
+            for (int q = 0; q < G; ++q)
+                acc[q][T2] = __builtin_amdgcn_mfma_f64_16x16x4f64(pf[T2 * V + t], x[q][t], acc[q][T2], 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < G; ++q)
+#pragma unroll
+        for (int t = 0; t < V; ++t) r[q][t] = acc[q][t / 4][t % 4];
+}
+
+// A tip child's state masks for the G site groups, and whether every site of
+// the wave is a single state or a gap (then the child's factor is a column
+// of P).
+template <int G, bool TL>
+__device__ __forceinline__ bool tip_masks(int idx, const int (&s)[G], int lane, int m_pad, uint32_t full,
+                                          const uint32_t* __restrict__ tips, const uint32_t* tl, uint32_t (&mk)[G]) {
+    bool simple = true;
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+        mk[q] = TL ? tl[idx * 16 * G + 16 * q + (lane & 15)] : tips[(size_t)idx * m_pad + s[q]];
+        simple = simple && (mk[q] == full || __popc(mk[q]) == 1);
+    }
+    return __all(simple);
+}
+
+// One child's factor in a shape-specialised op: a tip's P columns (col) or
+// products with its state indicators, a stored or carried partial's products.
+template <int K, typename T, int C, int V, int G>
+__device__ __forceinline__ void mfma_factor(bool col, const double (&cols)[G][V], const uint32_t (&mk)[G], int node,
+                                            int cat, int lane, int g, const double* __restrict__ Pm,
+                                            const double (&pf)[(V > 4 ? 2 : 1) * V], const T (&xr)[G][V],
+                                            const double (&cA)[G][V], const double* cB, double (&r)[G][V]) {
+    if constexpr (K == SRC_TIP) {
+        if (col) {
+#pragma unroll
+            for (int q = 0; q < G; ++q)
+#pragma unroll
+                for (int t = 0; t < V; ++t) r[q][t] = cols[q][t];
+            return;
+        }
+        double pt[(V > 4 ? 2 : 1) * V], x[G][V];
+        load_pm<C, V>(node, cat, lane, Pm, pt);
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+#pragma unroll
+            for (int t = 0; t < V; ++t) x[q][t] = (mk[q] >> (4 * t + g)) & 1u ? 1.0 : 0.0;
+        mfma_products<V, G>(pt, x, r);
+    } else if constexpr (K == SRC_CLV) {
+        double x[G][V];
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+#pragma unroll
+            for (int t = 0; t < V; ++t) x[q][t] = to_d(xr[q][t]);
+        mfma_products<V, G>(pf, x, r);
+    } else if constexpr (K == SRC_CARRY_A) {
+        mfma_products<V, G>(pf, cA, r);
+    } else {
+        double x[G][V];
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+#pragma unroll
+            for (int t = 0; t < V; ++t) x[q][t] = cB[(q * V + t) * 64];
+        mfma_products<V, G>(pf, x, r);
+    }
+}
+
+// An op of child shape (LK, RK) (kind order, RK != SRC_NONE): the tip masks
+// are read first, then every load of the op (P operands, stored partials and
+// their scalers, tip columns) is issued before the first product, so the two
+// children's memory latencies overlap.  val = left factor x right factor and
+// sc = the children's scalers, as the generic op computes them.
+template <int LK, int RK, typename T, int C, int V, int G, bool TL>
+__device__ __forceinline__ void mfma_shape(const Op& op, int cat, const int (&s)[G], int sg0, int n_sg, int lane,
+                                           int g, int m_pad, int S, uint32_t full, const uint32_t* __restrict__ tips,
+                                           const uint32_t* tl, const T* clv, size_t slot_stride, const int* scl,
+                                           const double* __restrict__ Pm, const double* __restrict__ PTm,
+                                           const double (&cA)[G][V], const int (&sA)[G], const double* cB,
+                                           const int (&sB)[G], double (&val)[G][V], int (&sc)[G]) {
+    constexpr int TNV = (V > 4 ? 2 : 1) * V;
+    uint32_t mkL[G], mkR[G];
+    bool colL = false, colR = false;
+    if constexpr (LK == SRC_TIP) colL = tip_masks<G, TL>(op.l_idx, s, lane, m_pad, full, tips, tl, mkL);
+    if constexpr (RK == SRC_TIP) colR = tip_masks<G, TL>(op.r_idx, s, lane, m_pad, full, tips, tl, mkR);
+    double pL[TNV], pR[TNV];
+    if constexpr (LK != SRC_TIP) load_pm<C, V>(op.l_node, cat, lane, Pm, pL);
+    T xL[G][V], xR[G][V];
+    const auto load_stored = [&](int idx, T (&xr)[G][V]) {
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            const T* src = clv + (size_t)idx * slot_stride + ((size_t)cat * n_sg + sg0 + q) * V * 64 + lane;
+#pragma unroll
+            for (int t = 0; t < V; ++t) xr[q][t] = src[t * 64];
+            sc[q] += scl[((size_t)idx * C + cat) * m_pad + s[q]];
+        }
+    };
+    if constexpr (LK == SRC_CLV) load_stored(op.l_idx, xL);
+    if constexpr (RK == SRC_CLV) load_stored(op.r_idx, xR);
+    double gL[G][V], gR[G][V];
+    const auto tip_columns = [&](int node, const uint32_t (&mk)[G], double (&cols)[G][V]) {
+        const double* base = PTm + ((size_t)node * C + cat) * (S + 1) * (4 * V) + g;
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            const int col = mk[q] == full ? S : __ffs(mk[q]) - 1;
+            const double* pt = base + (size_t)col * (4 * V);
+#pragma unroll
+            for (int t = 0; t < V; ++t) cols[q][t] = pt[4 * t];
+        }
+    };
+    if constexpr (LK == SRC_TIP)
+        if (colL) tip_columns(op.l_node, mkL, gL);
+    if constexpr (RK == SRC_TIP)
+        if (colR) tip_columns(op.r_node, mkR, gR);
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+        if constexpr (LK == SRC_CARRY_A || RK == SRC_CARRY_A) sc[q] += sA[q];
+        if constexpr (LK == SRC_CARRY_B || RK == SRC_CARRY_B) sc[q] += sB[q];
+    }
+    mfma_factor<LK, T, C, V, G>(colL, gL, mkL, op.l_node, cat, lane, g, Pm, pL, xL, cA, cB, val);
+    if constexpr (RK != SRC_TIP) load_pm<C, V>(op.r_node, cat, lane, Pm, pR);
+    double r[G][V];
+    mfma_factor<RK, T, C, V, G>(colR, gR, mkR, op.r_node, cat, lane, g, Pm, pR, xR, cA, cB, r);
+#pragma unroll
+    for (int q = 0; q < G; ++q)
+#pragma unroll
+        for (int t = 0; t < V; ++t) val[q][t] *= r[q][t];
+}
+
 // grid = (m_pad / (16 G), n_programs), block = 64 * C: wave c computes
 // category c of the G 16-site groups starting at group G * x of site block
 // x (blockIdx.x in a plain launch; the tail-split mapping of prune4_kernel
@@ -939,6 +1103,7 @@ __global__ __launch_bounds__(64 * C, (V <= 5 ? 3 : 1)) void prune_mfma_kernel(
     __shared__ int xsc[C][16 * G];
     __shared__ double xroot[C][16 * G];
     __shared__ uint32_t tl[TL ? kMfmaTipLdsTaxa * 16 * G : 1];
+    __shared__ double xb[C][G * V * 64];    // each wave's carry B (LDS keeps the registers for the products)
     // work mapping as in prune4_kernel (plain or tail-split launch)
     const int y = blockIdx.y;
     int bx = blockIdx.x, prog = y;
@@ -971,40 +1136,52 @@ __global__ __launch_bounds__(64 * C, (V <= 5 ? 3 : 1)) void prune_mfma_kernel(
     const size_t slot_stride = (size_t)C * m_pad * 4 * V;
     // carried results stay in FP64 and are rescaled lazily, as in the DNA
     // walker; only stored partials are rounded to T
-    double cA[G][V], cB[G][V];
+    double cA[G][V];
     int sA[G], sB[G];
+    double* const cB = &xb[cat][lane];      // carry B: (group q, slot t) at cB[64 (q V + t)]
 #pragma unroll
     for (int q = 0; q < G; ++q) {
 #pragma unroll
-        for (int t = 0; t < V; ++t) { cA[q][t] = 0.0; cB[q][t] = 0.0; }
+        for (int t = 0; t < V; ++t) { cA[q][t] = 0.0; cB[(q * V + t) * 64] = 0.0; }
         sA[q] = 0;
         sB[q] = 0;
     }
     double lnl[G];
 #pragma unroll
     for (int q = 0; q < G; ++q) lnl[q] = 0.0;
+    Op nxt = ops[k0];
     for (int k = k0; k < k1; ++k) {
-        const Op op = ops[k];
+        const Op op = nxt;
+        nxt = ops[k + 1];     // in flight during this op (an op buffer ends with one pad op)
         int sc[G];
 #pragma unroll
         for (int q = 0; q < G; ++q) sc[q] = 0;
-        mfma_d4 acc[G][2];
         double val[G][V];
-        mfma_child<T, C, V, G, TL>(op.l_kind, op.l_idx, op.l_node, cat, s, sg0, n_sg, lane, g, m_pad, S, full, tips, tl,
-                                   clv,
-                               slot_stride, scl, Pm, PTm, cA, sA, cB, sB, acc, sc);
-#pragma unroll
-        for (int q = 0; q < G; ++q)
-#pragma unroll
-            for (int t = 0; t < V; ++t) val[q][t] = acc[q][t / 4][t % 4];
-        if (op.r_kind != SRC_NONE) {
-            mfma_child<T, C, V, G, TL>(op.r_kind, op.r_idx, op.r_node, cat, s, sg0, n_sg, lane, g, m_pad, S, full,
-                                       tips, tl,
-                                   clv, slot_stride, scl, Pm, PTm, cA, sA, cB, sB, acc, sc);
+#define PHYLO_MFMA_SHAPE(LK, RK)                                                                                     \
+    mfma_shape<LK, RK, T, C, V, G, TL>(op, cat, s, sg0, n_sg, lane, g, m_pad, S, full, tips, tl, clv, slot_stride, \
+                                       scl, Pm, PTm, cA, sA, cB, sB, val, sc)
+        if (op.flags & OP_SH_AT) PHYLO_MFMA_SHAPE(SRC_CARRY_A, SRC_TIP);
+        else if (op.flags & OP_SH_TT) PHYLO_MFMA_SHAPE(SRC_TIP, SRC_TIP);
+        else if (op.flags & OP_SH_AB) PHYLO_MFMA_SHAPE(SRC_CARRY_A, SRC_CARRY_B);
+        else if (op.flags & OP_SH_AC) PHYLO_MFMA_SHAPE(SRC_CARRY_A, SRC_CLV);
+        else {
+#undef PHYLO_MFMA_SHAPE
+            // the other shapes: one child at a time, kinds read at run time
+            mfma_d4 acc[G][2];
+            mfma_child<T, C, V, G, TL>(op.l_kind, op.l_idx, op.l_node, cat, s, sg0, n_sg, lane, g, m_pad, S, full,
+                                       tips, tl, clv, slot_stride, scl, Pm, PTm, cA, sA, cB, sB, acc, sc);
 #pragma unroll
             for (int q = 0; q < G; ++q)
 #pragma unroll
-                for (int t = 0; t < V; ++t) val[q][t] *= acc[q][t / 4][t % 4];
+                for (int t = 0; t < V; ++t) val[q][t] = acc[q][t / 4][t % 4];
+            if (op.r_kind != SRC_NONE) {
+                mfma_child<T, C, V, G, TL>(op.r_kind, op.r_idx, op.r_node, cat, s, sg0, n_sg, lane, g, m_pad, S,
+                                           full, tips, tl, clv, slot_stride, scl, Pm, PTm, cA, sA, cB, sB, acc, sc);
+#pragma unroll
+                for (int q = 0; q < G; ++q)
+#pragma unroll
+                    for (int t = 0; t < V; ++t) val[q][t] *= acc[q][t / 4][t % 4];
+            }
         }
         // Lazy rescale: only when a site's largest state falls below 2^-256
         // (a site's states are spread over the lane groups, so its maximum
@@ -1046,7 +1223,7 @@ __global__ __launch_bounds__(64 * C, (V <= 5 ? 3 : 1)) void prune_mfma_kernel(
             }
             if (op.flags & OP_TO_B) {
 #pragma unroll
-                for (int t = 0; t < V; ++t) cB[q][t] = val[q][t];
+                for (int t = 0; t < V; ++t) cB[(q * V + t) * 64] = val[q][t];
                 sB[q] = sc[q];
             } else {
 #pragma unroll
@@ -1322,8 +1499,9 @@ void order_children(std::vector<Op>& ops) {
 }
 
 // Copies a program's ops to the device, children in kind order, followed by
-// one zero op: the DNA walker reads the op after the one it runs (up to
-// ops[end] of the last program) to load its P matrices early.
+// one zero op: the walkers read the op after the one they run (up to
+// ops[end] of the last program), the DNA walker to load its P matrices
+// early, the matrix walker to have the descriptor at hand.
 hipError_t upload_ops(Op*& d, size_t& have, std::vector<Op>& ops, hipStream_t st) {
     static const Op pad{};
     order_children(ops);
