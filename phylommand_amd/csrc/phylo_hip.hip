@@ -576,6 +576,14 @@ inline size_t tip_lds_bytes(int n_taxa) { return (size_t)n_taxa * 64; }
 
 // Plain launch (prog_main < 0): grid = (m_pad / 64, n_programs); workgroup
 // (x, y) runs ops[prog_begin[y] .. prog_begin[y+1]) over site block x.
+// Program-major launch (prog_main = -2, n_main site blocks, n_tail
+// programs, blk0 programs a workgroup; 1-D grid of 8 ceil(n_main / 8)
+// ceil(n_tail / blk0) workgroups): workgroup i runs on XCD i % 8
+// (workgroups are dealt to the XCDs in turn), and XCD k's workgroups take
+// site blocks k, k + 8, ... each with every group of blk0 consecutive
+// programs in turn, so the programs of one site block read its stored
+// partials through one L2.  Program p over block b writes
+// partial[p n_main + b], as a plain launch does.
 // Split launch (prog_main >= 0, grid.y = 1; build_split_programs): workgroups
 // x < n_main run program prog_main over site block blk0 + x; the others run
 // piece program prog_piece + (x - n_main) / n_tail over site block
@@ -612,6 +620,11 @@ __global__ __launch_bounds__(64 * C) void prune4_kernel(
             prog = prog_piece + r / n_tail;
             has_root = false;
         }
+    } else if (prog_main == -2) {
+        const int slot = (int)blockIdx.x >> 3, n_pg = (n_tail + blk0 - 1) / blk0;
+        prog = slot % n_pg * blk0;
+        bx = slot / n_pg * 8 + ((int)blockIdx.x & 7);
+        if (bx >= n_main) return;        // the last group's padding (uniform per workgroup)
     }
     const int s = bx * 64 + lane;
     if (TL) {
@@ -625,6 +638,26 @@ __global__ __launch_bounds__(64 * C) void prune4_kernel(
             reinterpret_cast<uint32_t*>(tl)[i] = (w << 4) & 0xF0F0F0F0u;
         }
         __syncthreads();
+    }
+    if constexpr (!TL) {
+        if (prog_main == -2) {
+            // blk0 programs of one site block, one after another
+            const int p_end = min(prog + blk0, n_tail);
+            for (int p = prog; p < p_end; ++p) {
+                double lc;
+                int sc;
+                walk4<T, C, false>(ops, prog_begin[p], prog_begin[p + 1], tips, tl, tipv, clv, scl, P, roots, m_pad,
+                                   s, lane, cat, xb, lc, sc);
+                const double lnl = combine_categories<C, true>(lc, sc, cat, lane, xroot, xsc);
+                const bool live = s < m && cat == 0;
+                if (live && site_lnl) site_lnl[(size_t)p * m + s] = lnl;
+                if (cat == 0) {
+                    const double v = wave_sum(live ? (site_w ? site_w[s] : 1.0) * lnl : 0.0);
+                    if (lane == 0) partial[(size_t)p * n_main + bx] = v;
+                }
+            }
+            return;
+        }
     }
     double lc;
     int sc;
@@ -2038,6 +2071,17 @@ void launch_prune4(phylo_ctx* c, dim3 grid, double* site_lnl) {
                                tips, clv, c->d_scl, c->d_P, c->d_roots, sw, site_lnl, c->d_partial, c->m, c->m_pad,
                                c->n_taxa, (int)grid.x, -1, 0, 0, 0, 1, nullptr, nullptr);
         }
+    } else if (grid.y > 1) {
+        // many programs (an NNI neighbourhood): program-major, grouped by
+        // XCD, four programs a workgroup while that leaves 8 workgroups per CU
+        const int nblk = (int)grid.x, n_prog = (int)grid.y, nb8 = (nblk + 7) / 8 * 8;
+        const long fill = 8L * (c->n_cu > 0 ? c->n_cu : 256);
+        int ppw = 4;
+        while (ppw > 1 && (long)nb8 * ((n_prog + ppw - 1) / ppw) < fill) ppw /= 2;
+        const int n_pg = (n_prog + ppw - 1) / ppw;
+        hipLaunchKernelGGL((prune4_kernel<T, C, false>), dim3(nb8 * n_pg), dim3(64 * C), 0,
+                           c->stream, c->run_ops, c->run_prog, tips, clv, c->d_scl, c->d_P, c->d_roots, sw, site_lnl,
+                           c->d_partial, c->m, c->m_pad, c->n_taxa, nblk, -2, ppw, 0, 0, n_prog, nullptr, nullptr);
     } else {
         hipLaunchKernelGGL((prune4_kernel<T, C, false>), grid, dim3(64 * C), 0, c->stream, c->run_ops, c->run_prog,
                            tips, clv, c->d_scl, c->d_P, c->d_roots, sw, site_lnl, c->d_partial, c->m, c->m_pad,
