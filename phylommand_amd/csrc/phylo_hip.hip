@@ -34,6 +34,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/phylo_hip.h"
@@ -3166,7 +3167,27 @@ void build_work_lists(phylo_batch* b, int bpi, const std::vector<int>& prog_len)
     b->items1.clear();
     b->items2.clear();
     for (auto& x : c1) b->items1.push_back(x.w);
-    for (auto& x : c2) b->items2.push_back(x.w);
+    // Workgroup g runs on XCD g mod 8 (workgroups are dealt to the XCDs in
+    // turn) and the grid is a multiple of 8, so item j runs on XCD j mod 8.
+    // The variants of one (alignment, site block) are dealt to one XCD, in
+    // cost order there, so those running side by side share the block's
+    // stored partials in that XCD's L2.
+    std::vector<std::vector<WorkItem>> q(8);
+    std::unordered_map<long, int> xcd_of;
+    for (const auto& x : c2) {
+        const long key = (long)x.w.inst << 32 | (unsigned)x.w.block;
+        const auto it = xcd_of.emplace(key, (int)(xcd_of.size() % 8)).first;
+        q[it->second].push_back(x.w);
+    }
+    size_t next[8] = {};
+    for (size_t j = 0; j < c2.size(); ++j) {
+        int x = (int)(j % 8);
+        if (next[x] == q[x].size()) {    // this XCD's share is done: take from the longest remaining
+            for (int k = 0; k < 8; ++k)
+                if (q[k].size() - next[k] > q[x].size() - next[x]) x = k;
+        }
+        b->items2.push_back(q[x][next[x]++]);
+    }
     for (auto& s : b->seg2) s += base2;
     b->seg2.front() = base2;
 }
