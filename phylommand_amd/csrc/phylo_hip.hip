@@ -1127,8 +1127,11 @@ constexpr int kMfmaTipLdsTaxa = 128;
 
 // TL (n_taxa <= kMfmaTipLdsTaxa): the workgroup first copies the tip masks of
 // its 16 G sites into LDS (tl[16 G t + site]).
-template <typename T, int C, int V, int G, bool TL>
-__global__ __launch_bounds__(64 * C, (V <= 5 ? 3 : 1)) void prune_mfma_kernel(
+// MP: the program-major launch of prune4_kernel (n_main site blocks, n_tail
+// programs, blk0 of them a workgroup, grouped by XCD), one program after
+// another in the workgroup.
+template <typename T, int C, int V, int G, bool TL, bool MP>
+__device__ __forceinline__ void prune_mfma_walk(
     const Op* __restrict__ ops, const int* __restrict__ prog_begin, const uint32_t* __restrict__ tips, T* clv,
     int* scl, const double* __restrict__ Pm, const double* __restrict__ PTm, const double* __restrict__ roots,
     const double* __restrict__ site_w, double* __restrict__ site_lnl, double* __restrict__ partial, int m, int m_pad,
@@ -1152,6 +1155,11 @@ __global__ __launch_bounds__(64 * C, (V <= 5 ? 3 : 1)) void prune_mfma_kernel(
             prog = prog_piece + r / n_tail;
             has_root = false;
         }
+    } else if (MP) {
+        const int slot = (int)blockIdx.x >> 3, n_pg = (n_tail + blk0 - 1) / blk0;
+        prog = slot % n_pg * blk0;
+        bx = slot / n_pg * 8 + ((int)blockIdx.x & 7);
+        if (bx >= n_main) return;        // the last group's padding (uniform per workgroup)
     }
     if (TL) {
         const uint32_t* blk = tips + (size_t)bx * 16 * G;
@@ -1166,7 +1174,10 @@ __global__ __launch_bounds__(64 * C, (V <= 5 ? 3 : 1)) void prune_mfma_kernel(
     int s[G];
 #pragma unroll
     for (int q = 0; q < G; ++q) s[q] = (sg0 + q) * 16 + j;
-    const int k0 = prog_begin[prog], k1 = prog_begin[prog + 1];
+    const int p_end = MP ? min(prog + blk0, n_tail) : prog + 1;
+    int p = prog;
+    do {    // MP: the workgroup's programs in turn; otherwise once
+    const int k0 = prog_begin[p], k1 = prog_begin[p + 1];
     const size_t slot_stride = (size_t)C * m_pad * 4 * V;
     // carried results stay in FP64 and are rescaled lazily, as in the DNA
     // walker; only stored partials are rounded to T
@@ -1312,11 +1323,46 @@ __global__ __launch_bounds__(64 * C, (V <= 5 ? 3 : 1)) void prune_mfma_kernel(
 #pragma unroll
     for (int q = 0; q < G; ++q) {
         const bool live = s[q] < m && cat == 0 && g == 0;
-        if (live && site_lnl) site_lnl[(size_t)y * m + s[q]] = lnl[q];
+        if (live && site_lnl) site_lnl[(size_t)(MP ? p : y) * m + s[q]] = lnl[q];
         if (live) sum += (site_w ? site_w[s[q]] : 1.0) * lnl[q];
     }
-    block_sum_store(sum, partial + (prog_main >= 0 ? (size_t)bx : (size_t)y * gridDim.x + blockIdx.x));
-    if (64 * C == kSumThreads && fold_counter) fold_total(partial, blk0 + n_main, fold_counter, fold_out);
+    if constexpr (MP) {
+        // only the category-0 wave holds site values: its sum is the block's
+        if (cat == 0) {
+            const double v = wave_sum(sum);
+            if (lane == 0) partial[(size_t)p * n_main + bx] = v;
+        }
+    } else {
+        block_sum_store(sum, partial + (prog_main >= 0 ? (size_t)bx : (size_t)y * gridDim.x + blockIdx.x));
+        if (64 * C == kSumThreads && fold_counter) fold_total(partial, blk0 + n_main, fold_counter, fold_out);
+    }
+    } while (MP && ++p < p_end);
+}
+
+template <typename T, int C, int V, int G, bool TL>
+__global__ __launch_bounds__(64 * C, (V <= 5 ? 3 : 1)) void prune_mfma_kernel(
+    const Op* __restrict__ ops, const int* __restrict__ prog_begin, const uint32_t* __restrict__ tips, T* clv,
+    int* scl, const double* __restrict__ Pm, const double* __restrict__ PTm, const double* __restrict__ roots,
+    const double* __restrict__ site_w, double* __restrict__ site_lnl, double* __restrict__ partial, int m, int m_pad,
+    int S, uint32_t full, int n_taxa, int n_main, int prog_main, int blk0, int prog_piece, int tail0, int n_tail,
+    unsigned* fold_counter, double* fold_out) {
+    prune_mfma_walk<T, C, V, G, TL, false>(ops, prog_begin, tips, clv, scl, Pm, PTm, roots, site_w, site_lnl, partial, m, m_pad, S, full,
+                                          n_taxa, n_main, prog_main, blk0, prog_piece, tail0, n_tail, fold_counter,
+                                          fold_out);
+}
+
+// An NNI neighbourhood's programs, program-major and grouped by XCD
+// (prune_mfma_walk's MP launch).
+template <typename T, int C, int V, int G>
+__global__ __launch_bounds__(64 * C, (V <= 5 ? 3 : 1)) void prune_mfma_nni_kernel(
+    const Op* __restrict__ ops, const int* __restrict__ prog_begin, const uint32_t* __restrict__ tips, T* clv,
+    int* scl, const double* __restrict__ Pm, const double* __restrict__ PTm, const double* __restrict__ roots,
+    const double* __restrict__ site_w, double* __restrict__ site_lnl, double* __restrict__ partial, int m, int m_pad,
+    int S, uint32_t full, int n_taxa, int n_main, int prog_main, int blk0, int prog_piece, int tail0, int n_tail,
+    unsigned* fold_counter, double* fold_out) {
+    prune_mfma_walk<T, C, V, G, false, true>(ops, prog_begin, tips, clv, scl, Pm, PTm, roots, site_w, site_lnl, partial, m, m_pad, S, full,
+                                          n_taxa, n_main, prog_main, blk0, prog_piece, tail0, n_tail, fold_counter,
+                                          fold_out);
 }
 
 // Compensated fixed-order sum of the per-workgroup partials of each program
@@ -2132,6 +2178,18 @@ void launch_mfma(phylo_ctx* c, dim3 grid, double* site_lnl) {
                                site_lnl, c->d_partial, c->m, c->m_pad, c->S, full, c->n_taxa, (int)grid.x, -1, 0, 0, 0,
                                1, nullptr, nullptr);
         }
+    } else if (grid.y > 1) {
+        // many programs (an NNI neighbourhood): program-major, grouped by XCD,
+        // as in launch_prune4
+        const int nblk = (int)grid.x, n_prog = (int)grid.y, nb8 = (nblk + 7) / 8 * 8;
+        const long fill = 8L * (c->n_cu > 0 ? c->n_cu : 256);
+        int ppw = 4;
+        while (ppw > 1 && (long)nb8 * ((n_prog + ppw - 1) / ppw) < fill) ppw /= 2;
+        const int n_pg = (n_prog + ppw - 1) / ppw;
+        hipLaunchKernelGGL((prune_mfma_nni_kernel<T, C, V, kMfmaGroups>), dim3(nb8 * n_pg), dim3(64 * C), 0,
+                           c->stream, c->run_ops, c->run_prog, tips, clv, c->d_scl, c->d_Pm, c->d_PTm, c->d_roots, sw,
+                           site_lnl, c->d_partial, c->m, c->m_pad, c->S, full, c->n_taxa, nblk, -2, ppw, 0, 0, n_prog,
+                           nullptr, nullptr);
     } else {
         hipLaunchKernelGGL((prune_mfma_kernel<T, C, V, kMfmaGroups, false>), grid, dim3(64 * C), 0, c->stream,
                            c->run_ops, c->run_prog, tips, clv, c->d_scl, c->d_Pm, c->d_PTm, c->d_roots, sw, site_lnl,
