@@ -194,6 +194,10 @@ class Context:
         if rc != PHYLO_OK:
             raise PhyloError(rc, "phylo_create failed (no GPU visible?)")
         self.h = h
+        # output cells of the per-evaluation calls, passed by reference (a
+        # numpy buffer's ctypes pointer costs a few microseconds a call)
+        self._tot = ctypes.c_double(0.0)
+        self._ms = ctypes.c_double(0.0)
         self._check(self.lib.phylo_set_precision(self.h, precision))
         self.n_sites = 0
         self.n_nodes = 0
@@ -276,10 +280,9 @@ class Context:
         self._check(self.lib.phylo_set_categories(self.h, len(rates), _dp(rates), _dp(weights)))
 
     def loglik(self, per_site: bool = False):
-        tot = np.zeros(1)
         site = np.zeros(self.n_sites) if per_site else None
-        self._check(self.lib.phylo_loglik(self.h, _dp(site) if per_site else None, _dp(tot)))
-        return (float(tot[0]), site) if per_site else float(tot[0])
+        self._check(self.lib.phylo_loglik(self.h, _dp(site) if per_site else None, ctypes.byref(self._tot)))
+        return (self._tot.value, site) if per_site else self._tot.value
 
     def node_partials(self, node: int, n_cat: int, n_states: int):
         clv = np.zeros((self.n_sites, n_cat, n_states))
@@ -337,9 +340,8 @@ class Context:
         self._check(self.lib.phylo_synchronize(self.h))
 
     def last_kernel_ms(self) -> float:
-        ms = np.zeros(1)
-        self._check(self.lib.phylo_last_kernel_ms(self.h, _dp(ms)))
-        return float(ms[0])
+        self._check(self.lib.phylo_last_kernel_ms(self.h, ctypes.byref(self._ms)))
+        return self._ms.value
 
     def simulate(self, n_sites: int, root_freq, seed: int = 1) -> np.ndarray:
         """States [n_nodes, n_sites] drawn down the context's tree and model."""
