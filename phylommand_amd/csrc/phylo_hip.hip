@@ -2180,12 +2180,12 @@ void launch_mfma(phylo_ctx* c, dim3 grid, double* site_lnl) {
         }
     } else if (grid.y > 1) {
         // many programs (an NNI neighbourhood): program-major, grouped by XCD,
-        // as in launch_prune4
+        // as in launch_prune4, but one program a workgroup: these ops are
+        // long enough that fewer, longer workgroups do not pay (one, two,
+        // four programs: 2.976, 2.997, 3.014 ms for a 122-variant
+        // neighbourhood of 64 taxa x 20,000 sites)
         const int nblk = (int)grid.x, n_prog = (int)grid.y, nb8 = (nblk + 7) / 8 * 8;
-        const long fill = 8L * (c->n_cu > 0 ? c->n_cu : 256);
-        int ppw = 4;
-        while (ppw > 1 && (long)nb8 * ((n_prog + ppw - 1) / ppw) < fill) ppw /= 2;
-        const int n_pg = (n_prog + ppw - 1) / ppw;
+        const int ppw = 1, n_pg = n_prog;
         hipLaunchKernelGGL((prune_mfma_nni_kernel<T, C, V, kMfmaGroups>), dim3(nb8 * n_pg), dim3(64 * C), 0,
                            c->stream, c->run_ops, c->run_prog, tips, clv, c->d_scl, c->d_Pm, c->d_PTm, c->d_roots, sw,
                            site_lnl, c->d_partial, c->m, c->m_pad, c->S, full, c->n_taxa, nblk, -2, ppw, 0, 0, n_prog,

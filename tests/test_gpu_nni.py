@@ -37,10 +37,11 @@ def test_nni_scores_match_full_reevaluation(oracle, n, m, C, S, mode, seed):
 
 @pytest.mark.parametrize("n,m,C,S,n_check,seed", [(40, 7000, 4, 4, None, 8), (64, 2100, 2, 20, 10, 9)])
 def test_nni_several_variants_a_workgroup_match_oracle(oracle, n, m, C, S, n_check, seed):
-    """Sizes whose neighbourhood launch fills the GPU with four variants a
-    workgroup (at 256 CUs), with a last group of two and a site-block count
-    that is not a multiple of 8.  The 20-state case checks a seeded sample
-    of variants that always includes the last group."""
+    """Program-major neighbourhood launches with a site-block count that is
+    not a multiple of 8: the DNA case fills the GPU with four variants a
+    workgroup (at 256 CUs) and ends with a group of two; the 20-state case
+    (one variant a workgroup) checks a seeded sample of variants that
+    always includes the last two."""
     print(f"nni several a workgroup n={n} m={m} C={C} S={S}", flush=True)
     kw = dict(rate_dist="gamma", freq_conc=2.0) if S > 4 else {}
     wl = synth.gtr_gamma_workload(n_tips=n, m=m, C=C, S=S, alpha=0.7, gap_frac=0.05, seed=seed, **kw)
